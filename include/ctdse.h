@@ -1,0 +1,204 @@
+/*
+ * ctdse.h -- C ABI of libsingleTDSE.so, the MI355X implementation of the CTDSE
+ * single-atom solver.
+ *
+ * Part 1 repeats the reference library's interface so that existing callers
+ * (1DTDSE/PythonTDSE.py, 1DTDSE/PythonCTDSE/structures.py, 1DTDSE/tests.py)
+ * load this library in place of the CPU one: the same symbol names and
+ * signatures, and structs laid out byte-for-byte like 1DTDSE/sources/structures.h
+ * (x86-64 natural alignment).  Ownership is unchanged: the library allocates
+ * every output array with malloc/calloc and the caller frees them through the
+ * destructors.
+ *
+ * Part 2 adds entry points the reference does not have: an error status in
+ * place of the reference's exit() calls, and a batched, device-resident
+ * interface (many instances in one persistent launch).
+ *
+ * Complex arrays are interleaved (re, im) doubles, as in the reference.
+ */
+#ifndef MMA_AMD_CTDSE_H
+#define MMA_AMD_CTDSE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Part 1: reference interface ---------------------------------------- */
+
+/* 1DTDSE/sources/structures.h:16-33 */
+typedef struct Efield_var {
+    double *tgrid;
+    double *Field;
+    double dt;
+    double omega;
+    double E0;
+    double phi;
+    double ton;
+    double toff;
+    int Nt;
+    int nc;
+} Efield_var;
+
+/* structures.h:39-42 */
+typedef struct trg_def {
+    double a;
+} trg_def;
+
+/* structures.h:48-53 */
+typedef struct analy_def {
+    double tprint;
+    int writewft;
+} analy_def;
+
+/* structures.h:59-71 */
+typedef struct output_print_def {
+    int Efield;
+    int FEfield;
+    int sourceterm;
+    int Fsourceterm;
+    int FEfieldM2;
+    int FsourceTermM2;
+    int PopTot;
+    int tgrid;
+    int omegagrid;
+    int PopInt;
+    int expval_x;
+} output_print_def;
+
+/* structures.h:80-118 */
+typedef struct inputs_def {
+    struct trg_def trg;
+    struct Efield_var Efield;
+    double Eguess;
+    double Einit;
+    double tmin;
+    int Nt;
+    int num_t;
+    double dt;
+    int num_r;
+    double dx;
+    double *psi0;
+    double *x;
+    int gauge;
+    double x_int;
+    struct analy_def analy;
+    int InterpByDTorNT;
+    int Ntinterp;
+    struct output_print_def Print;
+    double CV;
+    char precision[2];
+} inputs_def;
+
+/* structures.h:120-147 */
+typedef struct outputs_def {
+    double *tgrid;
+    double *Efield;
+    double *sourceterm;
+    double *omegagrid;
+    double *FEfield;
+    double *Fsourceterm;
+    double *FEfieldM2;
+    double *FsourcetermM2;
+    double *PopTot;
+    double *PopInt;
+    double *expval;
+    int Nt;
+    int Nomega;
+    double **psi;
+} outputs_def;
+
+/* Replaces 1DTDSE/sources/tools.c:31 (Initialise_grid_and_ground_state): allocates
+ * in->x (num_r+1) and in->psi0 (2*(num_r+1)), sets in->Einit.  Ground state by
+ * inverse iteration on the GPU, bit-identical to the reference's CPU result. */
+void Initialise_grid_and_ground_state(inputs_def *in);
+
+/* Replaces 1DTDSE/sources/singleTDSE.c:28 (call1DTDSE): refines the field, propagates
+ * on the GPU, computes spectra, fills *outputs (library-owned arrays). Mutates
+ * inputs->Efield.dt, ->Ntinterp and ->num_t exactly as the reference does. */
+void call1DTDSE(inputs_def *inputs, outputs_def *outputs);
+
+/* Replaces tools.c:423 (set_time_and_field): copies N samples into calloc'd arrays. */
+void set_time_and_field(inputs_def *in, double *time, double *field, int N);
+
+/* Replaces structures.c:92 (Set_all_prints); returned by value. */
+output_print_def Set_all_prints(void);
+
+/* Replaces structures.c:116 (Initialise_Printing_struct). */
+output_print_def Initialise_Printing_struct(void);
+
+/* Replaces tools.c:206 (window_analysis): photoelectron spectrum by the resolvent
+ * method, num_E energies; returns a malloc'd array of num_E doubles. */
+double *window_analysis(inputs_def inputs, double *psi, int num_E, double dE, double Estep, double E_start);
+
+/* Replaces 1DTDSE/sources/tools_fftw3.c:158 (GaborTransform): returns N_G malloc'd
+ * rows of N_freq doubles. */
+double **GaborTransform(double *signal, double dt, int N, int N_freq, int N_G, double t_min, double t_max,
+                        double a);
+
+/* Replace tools.c:408 (free_arr), structures.c:76 (free_mtrx),
+ * structures.c:15-49 (outputs_destructor) and structures.c:55-68 (inputs_destructor). */
+void free_arr(double *buffer);
+void free_mtrx(double **buf, int N_rows);
+void outputs_destructor(outputs_def *outputs);
+void inputs_destructor(inputs_def *in);
+
+/* ---- Part 2: additions (no counterpart in the reference) ----------------- */
+
+/* Status of the calling thread's last entry-point call.  The reference prints and
+ * calls exit() (tridiag.c:139-142, constants.c:74-75) or prints and carries on
+ * (tridiag.c:33,49); here every such condition sets a code and the void entry
+ * points return early, leaving the process alive. */
+enum ctdse_status {
+    CTDSE_OK = 0,
+    CTDSE_ERR_GS_NOT_CONVERGED = 1, /* 1000 inverse iterations without |dE| <= CV */
+    CTDSE_ERR_ZERO_PIVOT = 2,       /* a Thomas pivot vanished (tridiag.c:33,49)  */
+    CTDSE_ERR_ARGUMENT = 3,         /* NULL or short arrays, num_r < 2, tprint < dt, x_int outside the box */
+    CTDSE_ERR_UNSUPPORTED = 4,      /* velocity gauge (dead code in the reference), k1 <= 0 */
+    CTDSE_ERR_DEVICE = 5,           /* HIP runtime error, no GPU */
+    CTDSE_ERR_NONFINITE = 6         /* NaN/Inf in the field or the result */
+};
+int ctdse_last_error(void);
+const char *ctdse_last_error_message(void);
+
+/* Opaque per-geometry state on the device: grid, ground state and the
+ * step-independent Crank-Nicolson factor tables for one (num_r, dx, dt, a, x_int). */
+typedef struct ctdse_geometry ctdse_geometry;
+
+/* Builds a geometry.  psi0 == NULL: ground state by inverse iteration from
+ * (e_guess, cv), as Initialise_grid_and_ground_state; otherwise psi0 (host,
+ * 2*(num_r+1) interleaved doubles) is used as given.  Returns a ctdse_status. */
+int ctdse_geometry_create(int num_r, double dx, double dt, double a, double e_guess, double cv, double x_int,
+                          const double *psi0, ctdse_geometry **out);
+
+/* Ground-state energy, inverse iterations used (0 if psi0 was given) and the
+ * truncation depth (grid points on each side of a segment) of the segmented solve. */
+int ctdse_geometry_info(const ctdse_geometry *g, double *e_init, int *gs_iterations, int *halo_points);
+
+/* Copies x (num_r+1) and psi0 (2*(num_r+1)) to host memory. */
+int ctdse_geometry_ground_state(const ctdse_geometry *g, double *x, double *psi0);
+
+void ctdse_geometry_destroy(ctdse_geometry *g);
+
+/*
+ * Batched propagation: n_inst independent instances in one persistent launch on
+ * `stream` (a hipStream_t, NULL = default stream).  Every array below is DEVICE
+ * memory.  Instance i uses geometry geom[i] (host array of pointers) and runs
+ * n_steps[i] Crank-Nicolson steps (the reference's outputs.Nt).
+ *   field + field_off[i]: n_steps[i]+1 samples of the refined field (in_field of
+ *                          prop.c); sample n_steps[i] may be 0.
+ *   out + out_off[i]:     5 rows of n_steps[i] doubles each, in the order
+ *                          sourceterm, PopTot, PopInt, expval, Efield (the gated
+ *                          field), i.e. the exported samples 0..n_steps[i]-1.
+ * field_off and out_off are host arrays.  Returns a ctdse_status once the work
+ * queued on `stream` has finished.
+ */
+int ctdse_propagate_batch(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const double *field,
+                          const long long *field_off, double *out, const long long *out_off, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

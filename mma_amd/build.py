@@ -1,0 +1,58 @@
+"""Builds the native library mma_amd/lib/libsingleTDSE.so for gfx950 with hipcc.
+
+Usage: python -m mma_amd.build   (or mma_amd.build.build() from Python)
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+BUILD = os.path.join(HERE, "_build")
+LIBDIR = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("MMA_AMD_ARCH", "gfx950")
+
+COMMON = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I", INCLUDE, "-I", CSRC,
+          # no packed FP32 instructions (v_pk_*_f32) anywhere
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+# (source, extra flags): the setup and analysis code must not contract a*b+c into FMA,
+# the ground state's bitwise fixed point depends on it
+SOURCES = [
+    ("tdse_setup.hip", ["-ffp-contract=off"]),
+    ("tdse_analysis.hip", ["-ffp-contract=off"]),
+    ("tdse_propagate.hip", []),
+    ("ctdse_abi.cpp", ["-x", "hip"]),
+]
+
+LIBS = {"libsingleTDSE.so": [s for s, _ in SOURCES]}
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=False, resource_usage=False):
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = {}
+    for src, flags in SOURCES:
+        obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
+        extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage and src.endswith(".hip") else []
+        _run([HIPCC] + COMMON + flags + extra + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs[src] = obj
+    for lib, srcs in LIBS.items():
+        out = os.path.join(LIBDIR, lib)
+        tmp = out + ".tmp"
+        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [objs[s] for s in srcs], verbose)
+        os.replace(tmp, out)
+    return os.path.join(LIBDIR, "libsingleTDSE.so")
+
+
+if __name__ == "__main__":
+    print(build(verbose=True, resource_usage="--usage" in sys.argv))

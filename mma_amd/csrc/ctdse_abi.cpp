@@ -1,0 +1,636 @@
+// Host side of libsingleTDSE.so: the reference's C entry points and the batched
+// extension, orchestrating the HIP kernels of tdse_setup.hip / tdse_propagate.hip.
+// Every compute step runs on the GPU; the host only validates arguments, moves
+// data across the boundary and owns the reference's malloc/calloc output arrays.
+#include "ctdse.h"
+#include "tdse_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local int g_status = CTDSE_OK;
+thread_local std::string g_message;
+
+int set_status(int code, const std::string &msg)
+{
+    g_status = code;
+    g_message = msg;
+    return code;
+}
+int clear_status()
+{
+    g_status = CTDSE_OK;
+    g_message.clear();
+    return CTDSE_OK;
+}
+
+#define HIP_CHECK(expr)                                                                                   \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return set_status(CTDSE_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device allocation owned by the holder.
+struct DevMem {
+    void *p = nullptr;
+    DevMem() = default;
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    ~DevMem() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    hipError_t alloc(size_t bytes)
+    {
+        reset();
+        return hipMalloc(&p, bytes ? bytes : 8);
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+int device_ready()
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return set_status(CTDSE_ERR_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+    return CTDSE_OK;
+}
+
+// Persistent-kernel workspace (two psi buffers per workgroup slot), grown on demand.
+std::mutex g_launch_mutex;
+DevMem g_work;
+size_t g_work_bytes = 0;
+
+constexpr int kGsMaxIter = 1000;      // tridiag.c:139
+constexpr double kTruncTol = 1e-17;   // relative truncation of the segmented solve
+
+} // namespace
+
+struct ctdse_geometry {
+    TdseGeom dev{};
+    int num_r = 0;
+    double dx = 0, dt = 0, a = 0, x_int = 0;
+    double e_init = 0;
+    int gs_iters = 0;
+    DevMem x, psi0_nat, g, dpot, ib, psi0, chq, chg;
+};
+
+namespace {
+
+int run_ground_state(int num_r, double dx, double a, double e_guess, double cv, DevMem &x, DevMem &psi0, double *e_out,
+                     int *iters)
+{
+    const long long nx = num_r + 1;
+    DevMem work, job;
+    HIP_CHECK(x.alloc(nx * sizeof(double)));
+    HIP_CHECK(psi0.alloc(2 * nx * sizeof(double)));
+    HIP_CHECK(work.alloc(16 * (nx + 2) * sizeof(double)));
+    HIP_CHECK(job.alloc(sizeof(GsJob)));
+    GsJob J{};
+    J.num_r = num_r;
+    J.max_iter = kGsMaxIter;
+    J.dx = dx;
+    J.a = a;
+    J.e_guess = e_guess;
+    J.cv = cv;
+    J.x = x.as<double>();
+    J.psi0 = psi0.as<double>();
+    J.work = work.as<double>();
+    HIP_CHECK(hipMemcpy(job.p, &J, sizeof J, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(tdse_ground_state_kernel, dim3(1), dim3(64), 0, 0, job.as<GsJob>(), 1);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(&J, job.p, sizeof J, hipMemcpyDeviceToHost));
+    *e_out = J.energy;
+    *iters = J.iters;
+    if (J.status == 1)
+        return set_status(CTDSE_ERR_GS_NOT_CONVERGED, "ground state: |dE| > CV after 1000 inverse iterations");
+    if (J.status == 2) return set_status(CTDSE_ERR_ZERO_PIVOT, "ground state: zero pivot in the Thomas solve");
+    return CTDSE_OK;
+}
+
+int build_tables(ctdse_geometry *G)
+{
+    const int nx = G->num_r + 1;
+    const int nchunks = (nx + TDSE_CHUNK - 1) / TDSE_CHUNK;
+    const long long npad = tdse_tiled_size(nchunks);
+    DevMem piv, job;
+    HIP_CHECK(G->g.alloc(npad * sizeof(double)));
+    HIP_CHECK(G->dpot.alloc(npad * sizeof(double)));
+    HIP_CHECK(G->ib.alloc(npad * sizeof(double2)));
+    HIP_CHECK(G->psi0.alloc(npad * sizeof(double2)));
+    HIP_CHECK(G->chq.alloc((nchunks + 128) * sizeof(double2)));
+    HIP_CHECK(G->chg.alloc((nchunks + 128) * sizeof(double2)));
+    HIP_CHECK(piv.alloc(2 * (size_t)nx * sizeof(double)));
+    HIP_CHECK(job.alloc(sizeof(TableJob)));
+    TableJob J{};
+    J.num_r = G->num_r;
+    J.dx = G->dx;
+    J.dt = G->dt;
+    J.a = G->a;
+    J.x_int = G->x_int;
+    J.x = G->x.as<double>();
+    J.psi0_nat = G->psi0_nat.as<double>();
+    J.g = G->g.as<double>();
+    J.dpot = G->dpot.as<double>();
+    J.ib = G->ib.as<double2>();
+    J.psi0 = G->psi0.as<double2>();
+    J.chq = G->chq.as<double2>();
+    J.chg = G->chg.as<double2>();
+    J.pivot_work = piv.as<double>();
+    HIP_CHECK(hipMemcpy(job.p, &J, sizeof J, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(tdse_tables_grid_kernel, dim3(256), dim3(256), 0, 0, job.as<TableJob>());
+    hipLaunchKernelGGL(tdse_tables_pivot_kernel, dim3(1), dim3(64), 0, 0, job.as<TableJob>());
+    hipLaunchKernelGGL(tdse_tables_chunk_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, 0, job.as<TableJob>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(&J, job.p, sizeof J, hipMemcpyDeviceToHost));
+
+    const double rho = std::max(J.rho_fwd, J.rho_bwd);
+    if (!(rho > 0.0) || !(rho < 0.999) || !std::isfinite(rho))
+        return set_status(CTDSE_ERR_UNSUPPORTED, "Crank-Nicolson factor decay rate " + std::to_string(rho) +
+                                                     " too close to 1 for the segmented solve");
+    const int W = (int)std::ceil(std::log(kTruncTol) / std::log(rho));
+    const int H = (W + TDSE_CHUNK - 1) / TDSE_CHUNK;
+    const int own = 64 - 2 * H;
+    if (own < 8)
+        return set_status(CTDSE_ERR_UNSUPPORTED, "truncation depth " + std::to_string(W) +
+                                                     " points exceeds the segment size (dt/dx^2 too large)");
+    if (J.pint_lo < 0 || J.pint_hi > G->num_r)
+        return set_status(CTDSE_ERR_ARGUMENT, "x_int lies outside the grid");
+    TdseGeom &D = G->dev;
+    D.g = G->g.as<double>();
+    D.ib = G->ib.as<double2>();
+    D.dpot = G->dpot.as<double>();
+    D.psi0 = G->psi0.as<double2>();
+    D.chq = G->chq.as<double2>();
+    D.chg = G->chg.as<double2>();
+    D.nx = nx;
+    D.nchunks = nchunks;
+    D.npad = npad;
+    D.halo_lanes = H;
+    D.own_lanes = own;
+    D.nseg = (nchunks + own - 1) / own;
+    D.pint_lo = J.pint_lo;
+    D.pint_hi = J.pint_hi;
+    D.p0_lo = J.p0_lo;
+    D.p0_hi = J.p0_hi;
+    D.dx = G->dx;
+    D.dt = G->dt;
+    D.xmax = 0.5 * G->num_r * G->dx;
+    D.k36 = 36.0 * G->dt / (G->dx * G->dx);
+    D.pint0 = J.pint0;
+    return CTDSE_OK;
+}
+
+int geometry_from(int num_r, double dx, double dt, double a, double e_guess, double cv, double x_int,
+                  const double *psi0_host, const double *x_host, ctdse_geometry **out)
+{
+    if (!out) return set_status(CTDSE_ERR_ARGUMENT, "null output pointer");
+    *out = nullptr;
+    if (num_r < 2) return set_status(CTDSE_ERR_ARGUMENT, "num_r must be >= 2");
+    if (!(dx > 0) || !(dt > 0) || !(a > 0) || !std::isfinite(dx) || !std::isfinite(dt) || !std::isfinite(a))
+        return set_status(CTDSE_ERR_ARGUMENT, "dx, dt and the soft-Coulomb parameter a must be positive");
+    if (!(x_int > 0) || !std::isfinite(x_int)) return set_status(CTDSE_ERR_ARGUMENT, "x_int must be positive");
+    if (device_ready() != CTDSE_OK) return g_status;
+    auto *G = new ctdse_geometry;
+    G->num_r = num_r;
+    G->dx = dx;
+    G->dt = dt;
+    G->a = a;
+    G->x_int = x_int;
+    const size_t nx = num_r + 1;
+    int st = CTDSE_OK;
+    if (!psi0_host) {
+        st = run_ground_state(num_r, dx, a, e_guess, cv, G->x, G->psi0_nat, &G->e_init, &G->gs_iters);
+    } else {
+        hipError_t e = G->x.alloc(nx * sizeof(double));
+        if (e == hipSuccess) e = G->psi0_nat.alloc(2 * nx * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpy(G->psi0_nat.p, psi0_host, 2 * nx * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            if (x_host) {
+                e = hipMemcpy(G->x.p, x_host, nx * sizeof(double), hipMemcpyHostToDevice);
+            } else {
+                hipLaunchKernelGGL(tdse_xgrid_kernel, dim3((nx + 255) / 256), dim3(256), 0, 0, G->x.as<double>(), num_r,
+                                   dx);
+                e = hipGetLastError();
+            }
+        }
+        if (e != hipSuccess) st = set_status(CTDSE_ERR_DEVICE, std::string("geometry upload: ") + hipGetErrorString(e));
+    }
+    if (st == CTDSE_OK) st = build_tables(G);
+    if (st != CTDSE_OK && st != CTDSE_ERR_GS_NOT_CONVERGED) {
+        delete G;
+        return st;
+    }
+    *out = G;
+    return st;
+}
+
+struct InstSpec {
+    const ctdse_geometry *geom;
+    int n_steps;
+    const double *field;     // device
+    double *out;             // device, 5 rows of n_steps
+    double *snap = nullptr;  // device
+    int snap_every = 1, n_snap = 0;
+    double *psi_final = nullptr;
+};
+
+int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
+{
+    const int n = (int)spec.size();
+    if (n == 0) return CTDSE_OK;
+    std::vector<const ctdse_geometry *> geoms;
+    std::vector<TdseInst> insts(n);
+    long long stride = 0;
+    for (int i = 0; i < n; i++) {
+        const InstSpec &s = spec[i];
+        if (!s.geom || s.n_steps < 1 || !s.field || !s.out)
+            return set_status(CTDSE_ERR_ARGUMENT, "instance " + std::to_string(i) + ": bad descriptor");
+        int gi = (int)(std::find(geoms.begin(), geoms.end(), s.geom) - geoms.begin());
+        if (gi == (int)geoms.size()) geoms.push_back(s.geom);
+        TdseInst &I = insts[i];
+        I.geom = gi;
+        I.n_steps = s.n_steps;
+        I.field = s.field;
+        I.src = s.out;
+        I.ptot = s.out + s.n_steps;
+        I.pint = s.out + 2LL * s.n_steps;
+        I.expv = s.out + 3LL * s.n_steps;
+        I.efield = s.out + 4LL * s.n_steps;
+        I.snap = s.snap;
+        I.snap_every = std::max(1, s.snap_every);
+        I.n_snap = s.n_snap;
+        I.psi_final = s.psi_final;
+        stride = std::max(stride, s.geom->dev.npad);
+    }
+    // largest instances first (longest-processing-time order for the atomic queue)
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+        return (double)spec[p].geom->dev.nx * spec[p].n_steps > (double)spec[q].geom->dev.nx * spec[q].n_steps;
+    });
+    std::vector<TdseGeom> gdev(geoms.size());
+    for (size_t k = 0; k < geoms.size(); k++) gdev[k] = geoms[k]->dev;
+
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    int nw = 4;
+    if (const char *env = std::getenv("MMA_TDSE_WAVES")) nw = std::max(1, std::min(TDSE_MAX_WAVES, std::atoi(env)));
+    int per_cu = 2;
+    if (const char *env = std::getenv("MMA_TDSE_WG_PER_CU")) per_cu = std::max(1, std::atoi(env));
+    const int slots = std::max(1, std::min(n, prop.multiProcessorCount * per_cu));
+
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    const size_t need = (size_t)slots * 2 * stride * sizeof(double2);
+    if (need > g_work_bytes) {
+        HIP_CHECK(g_work.alloc(need));
+        g_work_bytes = need;
+    }
+    DevMem d_geom, d_inst, d_order, d_counter;
+    HIP_CHECK(d_geom.alloc(gdev.size() * sizeof(TdseGeom)));
+    HIP_CHECK(d_inst.alloc(insts.size() * sizeof(TdseInst)));
+    HIP_CHECK(d_order.alloc(order.size() * sizeof(int)));
+    HIP_CHECK(d_counter.alloc(sizeof(int)));
+    HIP_CHECK(hipMemcpyAsync(d_geom.p, gdev.data(), gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_inst.p, insts.data(), insts.size() * sizeof(TdseInst), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, d_counter.as<int>(), 1, 0);
+    hipLaunchKernelGGL(tdse_propagate_kernel, dim3(slots), dim3(64 * nw), 0, stream, d_geom.as<TdseGeom>(),
+                       d_inst.as<TdseInst>(), d_order.as<int>(), n, d_counter.as<int>(), g_work.as<double2>(), stride);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return CTDSE_OK;
+}
+
+double *host_copy(const double *dev, size_t n, size_t alloc_n)
+{
+    double *h = (double *)calloc(std::max<size_t>(alloc_n, 1), sizeof(double));
+    if (h && n && hipMemcpy(h, dev, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        free(h);
+        return nullptr;
+    }
+    return h;
+}
+
+int spectrum_to_host(const double *sig_dev, int n, double dt, double tmax, double **omega, double **F, double **M2)
+{
+    const int nc = n / 2 + 1;
+    DevMem X, om, f, m2;
+    HIP_CHECK(X.alloc(nc * sizeof(double2)));
+    HIP_CHECK(om.alloc(nc * sizeof(double)));
+    HIP_CHECK(f.alloc(2 * nc * sizeof(double)));
+    HIP_CHECK(m2.alloc(nc * sizeof(double)));
+    hipLaunchKernelGGL(tdse_dft_r2c_kernel, dim3((nc + 127) / 128), dim3(128), 0, 0, sig_dev, n, nc, X.as<double2>());
+    hipLaunchKernelGGL(tdse_spectrum_kernel, dim3((nc + 255) / 256), dim3(256), 0, 0, X.as<double2>(), nc, dt, tmax,
+                       om.as<double>(), f.as<double>(), m2.as<double>());
+    HIP_CHECK(hipGetLastError());
+    if (omega) *omega = host_copy(om.as<double>(), nc, nc);
+    *F = host_copy(f.as<double>(), 2 * nc, 2 * nc);
+    *M2 = host_copy(m2.as<double>(), nc, nc);
+    return CTDSE_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+// status hook for the analysis entry points (tdse_analysis.hip); not exported
+__attribute__((visibility("hidden"))) int ctdse_set_status_from_analysis(int code, const char *msg)
+{
+    return set_status(code, msg ? msg : "");
+}
+
+int ctdse_last_error(void) { return g_status; }
+const char *ctdse_last_error_message(void) { return g_message.c_str(); }
+
+int ctdse_geometry_create(int num_r, double dx, double dt, double a, double e_guess, double cv, double x_int,
+                          const double *psi0, ctdse_geometry **out)
+{
+    clear_status();
+    return geometry_from(num_r, dx, dt, a, e_guess, cv, x_int, psi0, nullptr, out);
+}
+
+int ctdse_geometry_info(const ctdse_geometry *g, double *e_init, int *gs_iterations, int *halo_points)
+{
+    clear_status();
+    if (!g) return set_status(CTDSE_ERR_ARGUMENT, "null geometry");
+    if (e_init) *e_init = g->e_init;
+    if (gs_iterations) *gs_iterations = g->gs_iters;
+    if (halo_points) *halo_points = g->dev.halo_lanes * TDSE_CHUNK;
+    return CTDSE_OK;
+}
+
+int ctdse_geometry_ground_state(const ctdse_geometry *g, double *x, double *psi0)
+{
+    clear_status();
+    if (!g) return set_status(CTDSE_ERR_ARGUMENT, "null geometry");
+    const size_t nx = g->num_r + 1;
+    if (x) HIP_CHECK(hipMemcpy(x, g->x.p, nx * sizeof(double), hipMemcpyDeviceToHost));
+    if (psi0) HIP_CHECK(hipMemcpy(psi0, g->psi0_nat.p, 2 * nx * sizeof(double), hipMemcpyDeviceToHost));
+    return CTDSE_OK;
+}
+
+void ctdse_geometry_destroy(ctdse_geometry *g) { delete g; }
+
+int ctdse_propagate_batch(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const double *field,
+                          const long long *field_off, double *out, const long long *out_off, void *stream)
+{
+    clear_status();
+    if (n_inst < 0 || (n_inst > 0 && (!geom || !n_steps || !field || !field_off || !out || !out_off)))
+        return set_status(CTDSE_ERR_ARGUMENT, "null batch argument");
+    if (device_ready() != CTDSE_OK) return g_status;
+    std::vector<InstSpec> spec(n_inst);
+    for (int i = 0; i < n_inst; i++) {
+        spec[i].geom = geom[i];
+        spec[i].n_steps = n_steps[i];
+        spec[i].field = field + field_off[i];
+        spec[i].out = out + out_off[i];
+    }
+    return propagate(spec, (hipStream_t)stream);
+}
+
+void Initialise_grid_and_ground_state(inputs_def *in)
+{
+    clear_status();
+    if (!in) {
+        set_status(CTDSE_ERR_ARGUMENT, "null inputs");
+        return;
+    }
+    if (in->num_r < 2 || !(in->dx > 0) || !(in->trg.a > 0)) {
+        set_status(CTDSE_ERR_ARGUMENT, "num_r >= 2, dx > 0 and trg.a > 0 are required");
+        return;
+    }
+    if (device_ready() != CTDSE_OK) return;
+    DevMem x, psi0;
+    double e = 0;
+    int it = 0;
+    const int st = run_ground_state(in->num_r, in->dx, in->trg.a, in->Eguess, in->CV, x, psi0, &e, &it);
+    if (st != CTDSE_OK && st != CTDSE_ERR_GS_NOT_CONVERGED && st != CTDSE_ERR_ZERO_PIVOT) return;
+    const size_t nx = in->num_r + 1;
+    in->x = host_copy(x.as<double>(), nx, nx);
+    in->psi0 = host_copy(psi0.as<double>(), 2 * nx, 2 * nx);
+    in->Einit = e;
+    if (!in->x || !in->psi0) set_status(CTDSE_ERR_DEVICE, "ground state download failed");
+}
+
+void call1DTDSE(inputs_def *inputs, outputs_def *outputs)
+{
+    clear_status();
+    if (!inputs || !outputs) {
+        set_status(CTDSE_ERR_ARGUMENT, "null inputs or outputs");
+        return;
+    }
+    inputs_def &in = *inputs;
+    if (!in.Efield.tgrid || !in.Efield.Field || in.Efield.Nt < 2) {
+        set_status(CTDSE_ERR_ARGUMENT, "field and time grid with at least 2 samples are required");
+        return;
+    }
+    if (!in.psi0 || !in.x) {
+        set_status(CTDSE_ERR_ARGUMENT, "psi0 and x are required (call Initialise_grid_and_ground_state)");
+        return;
+    }
+    if (in.gauge != 0) {
+        set_status(CTDSE_ERR_UNSUPPORTED, "only the length gauge is supported");
+        return;
+    }
+    if (!(in.dt > 0)) {
+        set_status(CTDSE_ERR_ARGUMENT, "dt must be positive");
+        return;
+    }
+    const int n_e = in.Efield.Nt;
+    for (int i = 0; i < n_e; i++)
+        if (!std::isfinite(in.Efield.Field[i])) {
+            set_status(CTDSE_ERR_NONFINITE, "non-finite field sample " + std::to_string(i));
+            return;
+        }
+    if (device_ready() != CTDSE_OK) return;
+
+    // singleTDSE.c:52-68, 86-93
+    const double field_dt = in.Efield.tgrid[1] - in.Efield.tgrid[0];
+    const double tmax = in.Efield.tgrid[n_e - 1] - in.Efield.tgrid[0];
+    int k1;
+    if (in.InterpByDTorNT == 1) {
+        k1 = in.Ntinterp + 1;
+    } else {
+        k1 = (int)std::floor(field_dt / in.dt);
+        in.Ntinterp = k1;
+        if (k1 != 1) k1++;
+    }
+    if (k1 < 1 || !(field_dt > 0)) {
+        set_status(CTDSE_ERR_UNSUPPORTED, "field refinement factor k1 < 1 (field dt smaller than dt?)");
+        return;
+    }
+    const double dt = field_dt / ((double)k1);
+    const int n2 = k1 * n_e;
+    const int nt_out = n2 + 2;
+    in.Efield.dt = dt;
+    in.num_t = (int)std::floor((2 * 3.14159265358979323846) / (0.057 * dt)) + 1;
+
+    int n_snap = 0, spd = 1;
+    if (in.analy.writewft) {
+        spd = (int)std::floor(in.analy.tprint / dt);
+        if (spd <= 0) {
+            set_status(CTDSE_ERR_ARGUMENT, "tprint < dt with writewft set");
+            return;
+        }
+        n_snap = nt_out / spd;
+    }
+
+    ctdse_geometry *G = nullptr;
+    int st = geometry_from(in.num_r, in.dx, dt, in.trg.a, in.Eguess, in.CV, in.x_int, in.psi0, in.x, &G);
+    if (st != CTDSE_OK) {
+        delete G;
+        return;
+    }
+    const size_t nx = in.num_r + 1;
+    DevMem fieldbuf, outbuf, snaps, tgrid;
+    auto fail_dev = [&](hipError_t e) {
+        set_status(CTDSE_ERR_DEVICE, std::string("call1DTDSE: ") + hipGetErrorString(e));
+        delete G;
+    };
+    hipError_t e = fieldbuf.alloc((size_t)(n2 + 3) * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(fieldbuf.p, 0, (size_t)(n2 + 3) * sizeof(double));
+    if (e != hipSuccess) return fail_dev(e);
+    if (k1 == 1) {
+        // FourInterp with k = 1 is the identity up to FFT rounding (tools_fftw3.c:24-68)
+        e = hipMemcpy(fieldbuf.p, in.Efield.Field, n_e * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail_dev(e);
+    } else {
+        DevMem raw, bins;
+        const int nc = n_e / 2 + 1;
+        e = raw.alloc(n_e * sizeof(double));
+        if (e == hipSuccess) e = bins.alloc(nc * sizeof(double2));
+        if (e == hipSuccess) e = hipMemcpy(raw.p, in.Efield.Field, n_e * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail_dev(e);
+        hipLaunchKernelGGL(tdse_dft_r2c_kernel, dim3((nc + 127) / 128), dim3(128), 0, 0, raw.as<double>(), n_e, nc,
+                           bins.as<double2>());
+        hipLaunchKernelGGL(tdse_c2r_interp_kernel, dim3((n2 + 127) / 128), dim3(128), 0, 0, bins.as<double2>(), nc,
+                           n_e, n2, fieldbuf.as<double>());
+        if ((e = hipGetLastError()) != hipSuccess) return fail_dev(e);
+    }
+    e = outbuf.alloc(5 * (size_t)nt_out * sizeof(double));
+    if (e == hipSuccess && n_snap > 0) e = snaps.alloc((size_t)n_snap * 2 * nx * sizeof(double));
+    if (e == hipSuccess && n_snap > 0) e = hipMemset(snaps.p, 0, (size_t)n_snap * 2 * nx * sizeof(double));
+    if (e == hipSuccess) e = tgrid.alloc((size_t)nt_out * sizeof(double));
+    if (e != hipSuccess) return fail_dev(e);
+
+    InstSpec s;
+    s.geom = G;
+    s.n_steps = nt_out;
+    s.field = fieldbuf.as<double>();
+    s.out = outbuf.as<double>();
+    if (n_snap > 0) {
+        s.snap = snaps.as<double>();
+        s.snap_every = spd;
+        s.n_snap = n_snap;
+    }
+    st = propagate({s}, 0);
+    if (st != CTDSE_OK) {
+        delete G;
+        return;
+    }
+    hipLaunchKernelGGL(tdse_tgrid_kernel, dim3(1), dim3(64), 0, 0, tgrid.as<double>(), nt_out, in.tmin, dt);
+
+    // outputs: library-owned host arrays sized as singleTDSE.c:95-100 (Nt + 2 with Nt = n2 + 1)
+    const size_t alloc_n = (size_t)n2 + 3;
+    const double *o = outbuf.as<double>();
+    outputs->tgrid = host_copy(tgrid.as<double>(), nt_out, alloc_n);
+    outputs->sourceterm = host_copy(o, nt_out, alloc_n);
+    outputs->PopTot = host_copy(o + nt_out, nt_out, alloc_n);
+    outputs->PopInt = host_copy(o + 2 * (size_t)nt_out, nt_out, alloc_n);
+    outputs->expval = host_copy(o + 3 * (size_t)nt_out, nt_out, alloc_n);
+    outputs->Efield = host_copy(o + 4 * (size_t)nt_out, nt_out, alloc_n);
+    outputs->Nt = nt_out;
+    double *dummy = nullptr;
+    st = spectrum_to_host(o + 4 * (size_t)nt_out, nt_out, dt, tmax, &dummy, &outputs->FEfield, &outputs->FEfieldM2);
+    free(dummy);
+    if (st == CTDSE_OK)
+        st = spectrum_to_host(o, nt_out, dt, tmax, &outputs->omegagrid, &outputs->Fsourceterm,
+                              &outputs->FsourcetermM2);
+    outputs->Nomega = nt_out / 2 + 1;
+    if (in.analy.writewft) {
+        outputs->psi = (double **)malloc(sizeof(double *) * std::max(n_snap, 1));
+        for (int r = 0; r < n_snap; r++)
+            outputs->psi[r] = host_copy(snaps.as<double>() + (size_t)r * 2 * nx, 2 * nx, 2 * nx);
+    }
+    delete G;
+}
+
+void set_time_and_field(inputs_def *in, double *time, double *field, int N)
+{
+    clear_status();
+    if (!in || N < 0 || (N > 0 && (!time || !field))) {
+        set_status(CTDSE_ERR_ARGUMENT, "set_time_and_field: bad arguments");
+        return;
+    }
+    in->Efield.tgrid = (double *)calloc(std::max(N, 1), sizeof(double));
+    in->Efield.Field = (double *)calloc(std::max(N, 1), sizeof(double));
+    for (int i = 0; i < N; i++) {
+        in->Efield.tgrid[i] = time[i];
+        in->Efield.Field[i] = field[i];
+    }
+}
+
+output_print_def Set_all_prints(void)
+{
+    output_print_def r;
+    r.Efield = r.FEfield = r.sourceterm = r.Fsourceterm = r.FEfieldM2 = r.FsourceTermM2 = 1;
+    r.PopTot = r.tgrid = r.omegagrid = r.PopInt = r.expval_x = 1;
+    return r;
+}
+
+output_print_def Initialise_Printing_struct(void)
+{
+    output_print_def r;
+    r.Efield = r.FEfield = r.sourceterm = r.Fsourceterm = r.FEfieldM2 = r.FsourceTermM2 = 0;
+    r.PopTot = r.tgrid = r.omegagrid = r.PopInt = r.expval_x = 0;
+    return r;
+}
+
+void free_arr(double *buffer) { free(buffer); }
+
+void free_mtrx(double **buf, int N_rows)
+{
+    if (!buf) return;
+    for (int i = 0; i < N_rows; i++) free(buf[i]);
+    free(buf);
+}
+
+void outputs_destructor(outputs_def *o)
+{
+    if (!o) return;
+    double **fields[] = {&o->tgrid, &o->omegagrid, &o->Efield, &o->sourceterm, &o->PopTot, &o->FEfield,
+                         &o->Fsourceterm, &o->FEfieldM2, &o->FsourcetermM2, &o->PopInt, &o->expval};
+    for (double **f : fields) {
+        free(*f);
+        *f = nullptr;
+    }
+}
+
+void inputs_destructor(inputs_def *in)
+{
+    if (!in) return;
+    free(in->psi0);
+    in->psi0 = nullptr;
+    free(in->x);
+    in->x = nullptr;
+    free(in->Efield.tgrid);
+    in->Efield.tgrid = nullptr;
+    free(in->Efield.Field);
+    in->Efield.Field = nullptr;
+}
+
+} // extern "C"

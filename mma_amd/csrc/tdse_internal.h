@@ -1,0 +1,108 @@
+// Internal device-side structures of libsingleTDSE.so (not part of the public ABI).
+//
+// Wavefunction/table layout in HBM ("chunk-interleaved tiles"): the grid is cut
+// into chunks of TDSE_CHUNK consecutive points; 64 consecutive chunks form a
+// tile stored point-major, chunk-minor:
+//     index(j) = (c / 64) * TDSE_TILE + (j % TDSE_CHUNK) * 64 + (c % 64),  c = j / TDSE_CHUNK
+// so the 64 lanes of a wave, each owning one chunk, read and write 64
+// consecutive complex values per instruction.  One zero tile precedes tile 0 and
+// one follows the last tile, so every chunk index in [-64, nchunks + 64) is
+// addressable and a wave never needs a bounds test on a load; points past the
+// end of the grid are padding rows (identity rows with zero data).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#ifndef TDSE_CHUNK
+#define TDSE_CHUNK 16
+#endif
+#define TDSE_WAVE 64
+#define TDSE_TILE (TDSE_WAVE * TDSE_CHUNK)
+#define TDSE_MAX_WAVES 4
+
+// storage offset of chunk c (c >= -64), row i of the chunk
+__host__ __device__ inline long long tdse_chunk_base(long long c) { return ((c >> 6) + 1) * TDSE_TILE + (c & 63); }
+
+__host__ __device__ inline long long tdse_index(long long j)
+{
+    return tdse_chunk_base(j / TDSE_CHUNK) + (j % TDSE_CHUNK) * 64;
+}
+
+// number of elements of a tiled array for nchunks chunks (with the two zero tiles)
+__host__ __device__ inline long long tdse_tiled_size(long long nchunks) { return ((nchunks + 63) / 64 + 2) * TDSE_TILE; }
+
+// Step-independent tables of one geometry (num_r, dx, dt, a, x_int), device pointers.
+// The Crank-Nicolson system of one step, scaled by 12, is
+//     B' u = A' psi,  B' = M2' + i H',  A' = M2' - i H',  M2' = tridiag(1, 10, 1),
+//     H' = tridiag(g_{j-1}, d_j, g_{j+1}),  g_j = -3 dt/dx^2 + dt V_j / 2,  d_j = 10 g_j + 36 dt/dx^2
+// (1DTDSE/sources/prop.c:138-190 times 12).  ib_j = 1/beta_j are the pivots of
+// the Thomas factorisation of B' (tridiag.c:26-70), which never change.
+struct TdseGeom {
+    const double *g;      // g_j, tiled
+    const double2 *ib;    // 1/beta_j, tiled (0 on padding rows)
+    const double *dpot;   // V'(x_j), tiled
+    const double2 *psi0;  // ground state, tiled
+    const double2 *chq;   // per chunk c at [c + 64]: product of q_j = -a'_j ib_j (forward transfer)
+    const double2 *chg;   // per chunk c at [c + 64]: product of h_j = -c'_j ib_j (backward transfer)
+    int nx;               // num_r + 1 grid points
+    int nchunks;          // ceil(nx / TDSE_CHUNK)
+    long long npad;       // tiled array length, tdse_tiled_size(nchunks)
+    int halo_lanes;       // H: truncation depth in chunks on each side of a segment
+    int own_lanes;        // 64 - 2H chunks owned per segment
+    int nseg;             // ceil(nchunks / own_lanes)
+    int pint_lo, pint_hi; // PopInt window [k1, k4] (prop.c:311-318)
+    int p0_lo, p0_hi;     // rows where |psi0| > 1e-30 max|psi0|
+    double dx, dt, xmax;  // x_j = j dx - xmax
+    double k36;           // 36 dt / dx^2
+    double pint0;         // PopInt at t = tmin (prop.c:86-93)
+};
+
+// One instance of a batch.
+struct TdseInst {
+    int geom;             // index into the geometry array
+    int n_steps;          // outputs.Nt of the reference: steps run and samples exported
+    const double *field;  // in_field, n_steps + 1 samples
+    double *src, *ptot, *pint, *expv;  // n_steps samples each
+    double *efield;       // gated field, n_steps samples (may be null)
+    double *snap;         // snapshots (natural order, interleaved), n_snap rows (may be null)
+    int snap_every, n_snap;
+    double *psi_final;    // final state, natural order (may be null)
+};
+
+// Ground-state job: inverse iteration in one lane (bit-identical to the CPU reference).
+struct GsJob {
+    int num_r, max_iter;
+    double dx, a, e_guess, cv;
+    double *x;        // num_r + 1
+    double *psi0;     // 2 (num_r + 1), natural order, interleaved
+    double *work;     // 16 (num_r + 2) doubles of scratch
+    double energy;
+    int iters, status;
+};
+
+// Table-builder job for one geometry.
+struct TableJob {
+    int num_r;
+    double dx, dt, a, x_int;
+    const double *x;        // natural order grid (from the ground-state job or the caller)
+    const double *psi0_nat; // natural order interleaved ground state
+    double *g, *dpot;       // tiled
+    double2 *ib, *psi0, *chq, *chg;  // chq/chg: nchunks + 128 entries, chunk c at c + 64
+    double *pivot_work;     // 2 (num_r + 1) scratch (natural-order 1/beta)
+    double rho_fwd, rho_bwd, pint0;
+    int pint_lo, pint_hi, p0_lo, p0_hi;
+};
+
+// kernels (defined in the .hip files)
+__global__ void tdse_ground_state_kernel(GsJob *jobs, int n_jobs);
+__global__ void tdse_tables_grid_kernel(TableJob *job);
+__global__ void tdse_tables_pivot_kernel(TableJob *job);
+__global__ void tdse_tables_chunk_kernel(TableJob *job);
+__global__ void tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
+                                      int *counter, double2 *work, long long work_stride);
+__global__ void tdse_fill_int_kernel(int *p, int n, int v);
+__global__ void tdse_xgrid_kernel(double *x, int num_r, double dx);
+__global__ void tdse_tgrid_kernel(double *tgrid, int n, double tmin, double dt);
+__global__ void tdse_dft_r2c_kernel(const double *sig, int n, int nbins, double2 *out);
+__global__ void tdse_spectrum_kernel(const double2 *dft, int nbins, double dt, double tmax, double *omega,
+                                     double *F, double *M2);
+__global__ void tdse_c2r_interp_kernel(const double2 *bins, int nbins, int n_in, int n_out, double *out);

@@ -1,0 +1,104 @@
+"""Synthetic driving fields and instance batches for the TDSE path.
+
+Units: atomic units inside the TDSE (the constants of 1DTDSE/sources/constants.c:26-43);
+SI for laser/beam parameters.  Nothing here is measured: these generators produce
+the field samples the TDSE consumes, in place of CUPRAD's output_field cube.
+"""
+import math
+
+import numpy as np
+
+# constants.c:26-43 (C set used by the TDSE)
+HBAR = 1.054571800e-34
+ALPHA_FINE = 1.0 / 137.035999139
+C_LIGHT = 299792458.0
+ELCHARGE = 1.602176565e-19
+ELMASS = 9.10938356e-31
+MU0 = 4.0 * math.pi * 1e-7
+EPS0 = 1.0 / (MU0 * C_LIGHT * C_LIGHT)
+R_BOHR = 4.0 * math.pi * EPS0 * HBAR * HBAR / (ELMASS * ELCHARGE * ELCHARGE)
+TIMEAU = (ELMASS * R_BOHR * R_BOHR) / HBAR
+EFIELDAU = HBAR * HBAR / (ELMASS * R_BOHR ** 3 * ELCHARGE)
+
+# Soft-Coulomb presets, constants.c:55-116
+SOFT_COULOMB_A = {"He": 0.6950, "Ne": 0.8161, "Ar": 1.1893, "Kr": 1.3676, "Xe": 1.6171}
+SOFT_COULOMB_E = {"He": -0.9036, "Ne": -0.7924, "Ar": -0.5792, "Kr": -0.5145, "Xe": -0.4458}
+
+
+def omega_au(wavelength_m):
+    """Angular frequency in a.u. of light of the given vacuum wavelength."""
+    return 2.0 * math.pi * C_LIGHT / wavelength_m * TIMEAU
+
+
+def field_au(intensity_w_cm2):
+    """Peak field in a.u. for a cycle-averaged intensity I = eps0 c E^2 / 2."""
+    e_si = math.sqrt(2.0 * intensity_w_cm2 * 1e4 / (C_LIGHT * EPS0))
+    return e_si / EFIELDAU
+
+
+def sin2_pulse(t, e0, omega, n_cycles, phase=0.0):
+    """E(t) = E0 sin^2(pi t/T) sin(omega t + phase) on [0, T], zero after; T = n_cycles 2pi/omega."""
+    T = n_cycles * 2.0 * math.pi / omega
+    t = np.asarray(t, dtype=np.float64)
+    env = np.where(t <= T, np.sin(math.pi * t / T) ** 2, 0.0)
+    return e0 * env * np.sin(omega * t + phase)
+
+
+def single_instance(n_cycles=10, wavelength=800e-9, intensity=1.5e14, dt=0.25):
+    """Single-instance case: 800 nm, sin^2 envelope, sampled at the TDSE step (k1 = 1)."""
+    w = omega_au(wavelength)
+    T = n_cycles * 2.0 * math.pi / w
+    n_e = int(math.ceil(T / dt)) + 1
+    t = np.arange(n_e) * dt
+    return t, sin2_pulse(t, field_au(intensity), w, n_cycles)
+
+
+# grid and atom shared by the TDSE workloads (dx = 0.1, x in [-800, 800], Ar)
+GRID = dict(num_r=16000, dx=0.1, a=SOFT_COULOMB_A["Ar"], e_guess=SOFT_COULOMB_E["Ar"], cv=1e-25, x_int=2.0)
+
+
+def gaussian_beam_batch(n_r=64, n_z=64, w0=100e-6, wavelength=800e-9, intensity=1.5e14, z_span=5e-3,
+                        n_cycles=10, dt=0.25):
+    """Uniform batch: fields E(r, z, t) of a focused Gaussian beam, following the
+    Gaussian-beam field of shared_python/mynumerics.py:369-379 (amplitude w0/w(z),
+    radial exp(-r^2/w^2), Gouy phase atan(z/zR), curvature phase k r^2 z/(2(zR^2+z^2)))
+    with a sin^2 envelope.  r in linspace(0, 2 w0, n_r), z in linspace(-z_span/2, z_span/2, n_z).
+    Returns t (n_e,), fields (n_z*n_r, n_e) ordered instance = kz*n_r + kr (MPI-TDSE.c:148)."""
+    w = omega_au(wavelength)
+    e0 = field_au(intensity)
+    T = n_cycles * 2.0 * math.pi / w
+    n_e = int(math.ceil(T / dt)) + 1
+    t = np.arange(n_e) * dt
+    k0 = 2.0 * math.pi / wavelength
+    z_r = math.pi * w0 ** 2 / wavelength
+    r = np.linspace(0.0, 2.0 * w0, n_r)
+    z = np.linspace(-z_span / 2, z_span / 2, n_z)
+    wz = w0 * np.sqrt(1.0 + (z / z_r) ** 2)
+    gouy = np.arctan(z / z_r)
+    curv = 0.5 * k0 * (r[None, :] ** 2) * (z / (z_r ** 2 + z ** 2))[:, None]
+    amp = (w0 / wz)[:, None] * np.exp(-(r[None, :] / wz[:, None]) ** 2)
+    phase = gouy[:, None] - curv
+    env = np.where(t <= T, np.sin(math.pi * t / T) ** 2, 0.0)
+    fields = (e0 * amp.reshape(-1, 1)) * env[None, :] * np.sin(w * t[None, :] + phase.reshape(-1, 1))
+    return t, fields
+
+
+def uneven_batch(n_inst=8192, seed=2024, dt=0.25, nr_range=(2000, 16000), cycles=(6, 20),
+                 intensity=(5e13, 3e14), wavelength=(800e-9, 1800e-9)):
+    """Uneven batch from numpy.random.Generator(PCG64(seed)): num_r = 2*U{nr_range},
+    cycles U{cycles}, intensity log-uniform, wavelength uniform; fields sampled at dt.
+    Returns a list of dicts with num_r, n_cycles, intensity, wavelength, t, field."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    half = rng.integers(nr_range[0], nr_range[1] + 1, size=n_inst)
+    cyc = rng.integers(cycles[0], cycles[1] + 1, size=n_inst)
+    inten = np.exp(rng.uniform(math.log(intensity[0]), math.log(intensity[1]), size=n_inst))
+    lam = rng.uniform(wavelength[0], wavelength[1], size=n_inst)
+    out = []
+    for i in range(n_inst):
+        w = omega_au(lam[i])
+        T = cyc[i] * 2.0 * math.pi / w
+        n_e = int(math.ceil(T / dt)) + 1
+        t = np.arange(n_e) * dt
+        out.append(dict(num_r=int(2 * half[i]), n_cycles=int(cyc[i]), intensity=float(inten[i]),
+                        wavelength=float(lam[i]), t=t, field=sin2_pulse(t, field_au(inten[i]), w, int(cyc[i]))))
+    return out

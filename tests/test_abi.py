@@ -1,0 +1,56 @@
+"""CPU checks of the drop-in boundary: the library loads, exports every symbol that
+include/ctdse.h declares, and its structs (C header and ctypes mirror) have the reference's
+byte layout (tests/golden/ref_struct_layout.json, probed from 1DTDSE/sources/structures.h)."""
+import ctypes
+import json
+import os
+import re
+import sys
+
+import pytest
+
+from conftest import ROOT, GOLDEN
+
+sys.path.insert(0, GOLDEN)
+import layout_probe  # noqa: E402
+
+
+def declared_functions(header):
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\**\s*\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while", "for", "return")))
+
+
+def test_header_declares_reference_entry_points():
+    names = declared_functions(os.path.join(ROOT, "include", "ctdse.h"))
+    for n in ["Initialise_grid_and_ground_state", "call1DTDSE", "set_time_and_field", "Set_all_prints",
+              "window_analysis", "GaborTransform", "free_arr", "free_mtrx", "outputs_destructor",
+              "inputs_destructor", "ctdse_last_error", "ctdse_propagate_batch"]:
+        assert n in names, n
+
+
+def test_library_exports_every_declared_symbol():
+    from mma_amd import library_path
+    if not os.path.exists(library_path()):
+        from mma_amd import build
+        build.build()
+    lib = ctypes.CDLL(library_path())
+    for n in declared_functions(os.path.join(ROOT, "include", "ctdse.h")):
+        assert hasattr(lib, n), n
+
+
+def test_struct_layout_matches_reference():
+    ref = json.load(open(os.path.join(GOLDEN, "ref_struct_layout.json")))
+    ours = layout_probe.probe(os.path.join(ROOT, "include"), "ctdse.h")
+    assert ours == ref
+
+
+def test_ctypes_mirror_layout_matches_reference():
+    from mma_amd.ctdse import structures as S
+    ref = json.load(open(os.path.join(GOLDEN, "ref_struct_layout.json")))
+    for name, spec in ref.items():
+        cls = getattr(S, name)
+        assert ctypes.sizeof(cls) == spec["size"], name
+        for f, off in spec["fields"].items():
+            assert getattr(cls, f).offset == off, (name, f)
