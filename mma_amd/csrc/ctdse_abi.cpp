@@ -67,7 +67,7 @@ int device_ready()
     return CTDSE_OK;
 }
 
-// Persistent-kernel workspace (two psi buffers per workgroup slot), grown on demand.
+// Persistent-kernel workspace (one psi buffer per workgroup slot, updated in place), grown on demand.
 std::mutex g_launch_mutex;
 DevMem g_work;
 size_t g_work_bytes = 0;
@@ -161,8 +161,7 @@ int build_tables(ctdse_geometry *G)
                                                      " too close to 1 for the segmented solve");
     const int W = (int)std::ceil(std::log(kTruncTol) / std::log(rho));
     const int H = (W + TDSE_CHUNK - 1) / TDSE_CHUNK;
-    const int own = 64 - 2 * H;
-    if (own < 8)
+    if (H > TDSE_MAX_HALO || TDSE_MAX_WAVES * 64 - 2 * H < 64)
         return set_status(CTDSE_ERR_UNSUPPORTED, "truncation depth " + std::to_string(W) +
                                                      " points exceeds the segment size (dt/dx^2 too large)");
     if (J.pint_lo < 0 || J.pint_hi > G->num_r)
@@ -178,8 +177,6 @@ int build_tables(ctdse_geometry *G)
     D.nchunks = nchunks;
     D.npad = npad;
     D.halo_lanes = H;
-    D.own_lanes = own;
-    D.nseg = (nchunks + own - 1) / own;
     D.pint_lo = J.pint_lo;
     D.pint_hi = J.pint_hi;
     D.p0_lo = J.p0_lo;
@@ -287,14 +284,37 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
     HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    int nw = 4;
-    if (const char *env = std::getenv("MMA_TDSE_WAVES")) nw = std::max(1, std::min(TDSE_MAX_WAVES, std::atoi(env)));
-    int per_cu = 2;
+    // Waves per workgroup: a segment is nw*64 chunks of which 2H are truncation halo; pick the
+    // nw that wastes least on the work-weighted geometries, mildly preferring occupancy
+    // (TDSE_WAVES_PER_CU waves fit on a CU).
+    int min_nw = 1;
+    for (const ctdse_geometry *g : geoms)
+        while (min_nw * 64 - 2 * g->dev.halo_lanes < 64) min_nw++;
+    int nw = std::max(4, min_nw);
+    double best = -1.0;
+    for (int w = std::max(4, min_nw); w <= TDSE_MAX_WAVES; w++) {
+        double used = 0.0, lanes = 0.0;
+        for (int i = 0; i < n; i++) {
+            const TdseGeom &d = spec[i].geom->dev;
+            const int own = w * 64 - 2 * d.halo_lanes;
+            const int nseg = (d.nchunks + own - 1) / own;
+            used += (double)d.nchunks * spec[i].n_steps;
+            lanes += (double)nseg * w * 64 * spec[i].n_steps;
+        }
+        const double occ = (double)((TDSE_WAVES_PER_CU / w) * w) / TDSE_WAVES_PER_CU;
+        const double score = (used / lanes) * std::pow(occ, 0.3);
+        if (score > best) {
+            best = score;
+            nw = w;
+        }
+    }
+    if (const char *env = std::getenv("MMA_TDSE_WAVES")) nw = std::max(min_nw, std::min(TDSE_MAX_WAVES, std::atoi(env)));
+    int per_cu = std::max(1, TDSE_WAVES_PER_CU / nw);
     if (const char *env = std::getenv("MMA_TDSE_WG_PER_CU")) per_cu = std::max(1, std::atoi(env));
     const int slots = std::max(1, std::min(n, prop.multiProcessorCount * per_cu));
 
     std::lock_guard<std::mutex> lock(g_launch_mutex);
-    const size_t need = (size_t)slots * 2 * stride * sizeof(double2);
+    const size_t need = (size_t)slots * stride * sizeof(double2);
     if (need > g_work_bytes) {
         HIP_CHECK(g_work.alloc(need));
         g_work_bytes = need;

@@ -13,11 +13,13 @@
 #include <hip/hip_runtime.h>
 
 #ifndef TDSE_CHUNK
-#define TDSE_CHUNK 16
+#define TDSE_CHUNK 8
 #endif
 #define TDSE_WAVE 64
 #define TDSE_TILE (TDSE_WAVE * TDSE_CHUNK)
-#define TDSE_MAX_WAVES 4
+#define TDSE_MAX_WAVES 8
+#define TDSE_WAVES_PER_CU 8
+#define TDSE_MAX_HALO 40  // largest truncation depth in chunks (halo saved in LDS per segment)  // occupancy of the propagation kernel (2 waves per SIMD)
 
 // storage offset of chunk c (c >= -64), row i of the chunk
 __host__ __device__ inline long long tdse_chunk_base(long long c) { return ((c >> 6) + 1) * TDSE_TILE + (c & 63); }
@@ -46,9 +48,7 @@ struct TdseGeom {
     int nx;               // num_r + 1 grid points
     int nchunks;          // ceil(nx / TDSE_CHUNK)
     long long npad;       // tiled array length, tdse_tiled_size(nchunks)
-    int halo_lanes;       // H: truncation depth in chunks on each side of a segment
-    int own_lanes;        // 64 - 2H chunks owned per segment
-    int nseg;             // ceil(nchunks / own_lanes)
+    int halo_lanes;       // H: truncation depth in chunks on each end of a workgroup segment
     int pint_lo, pint_hi; // PopInt window [k1, k4] (prop.c:311-318)
     int p0_lo, p0_hi;     // rows where |psi0| > 1e-30 max|psi0|
     double dx, dt, xmax;  // x_j = j dx - xmax

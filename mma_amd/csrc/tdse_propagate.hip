@@ -28,7 +28,6 @@
 namespace {
 
 constexpr int C = TDSE_CHUNK;
-constexpr int kWindow = 8;  // table loads a pass may issue ahead of their use
 
 __device__ inline double2 shfl_up2(double2 v, int d)
 {
@@ -43,6 +42,89 @@ __device__ inline double2 cmul(double2 a, double2 b)
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// ---- cross-lane moves in the VALU (DPP), 64-bit values as two 32-bit halves ----
+// Lanes without a source keep `old` (bound_ctrl off), so scans pad with the identity map.
+constexpr int kRowShr = 0x110, kRowShl = 0x100, kWaveShr1 = 0x138, kWaveShl1 = 0x130;
+
+template <int CTRL>
+__device__ inline double dpp(double v, double old)
+{
+    const long long b = __builtin_bit_cast(long long, v), o = __builtin_bit_cast(long long, old);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL>
+__device__ inline double2 dpp2(double2 v, double2 old)
+{
+    return make_double2(dpp<CTRL>(v.x, old.x), dpp<CTRL>(v.y, old.y));
+}
+__device__ inline double readlane(double v, int l)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double2 readlane2(double2 v, int l) { return make_double2(readlane(v.x, l), readlane(v.y, l)); }
+
+// (Q, z) <- (Q, z) o (Qo, zo): apply the map (Qo, zo) first, then (Q, z)
+__device__ inline void compose(double2 &Q, double2 &z, double2 Qo, double2 zo)
+{
+    z = make_double2(z.x + Q.x * zo.x - Q.y * zo.y, z.y + Q.x * zo.y + Q.y * zo.x);
+    Q = make_double2(Q.x * Qo.x - Q.y * Qo.y, Q.x * Qo.y + Q.y * Qo.x);
+}
+
+// Inclusive scan over the 64 lanes of the affine maps x_out = z + Q x_in, in lane order
+// (FWD: lane l composes lanes 0..l) or reverse order (lanes l..63).  Rows of 16 lanes
+// scan with DPP shifts; rows are then joined through the row totals read with readlane.
+template <bool FWD>
+__device__ inline void wave_scan(double2 &Q, double2 &z, int lane)
+{
+    const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+#define TDSE_SCAN_LEVEL(D)                                                                                   \
+    {                                                                                                        \
+        const double2 Qo = FWD ? dpp2<kRowShr + D>(Q, one) : dpp2<kRowShl + D>(Q, one);                      \
+        const double2 zo = FWD ? dpp2<kRowShr + D>(z, zero) : dpp2<kRowShl + D>(z, zero);                    \
+        compose(Q, z, Qo, zo);                                                                               \
+    }
+    TDSE_SCAN_LEVEL(1)
+    TDSE_SCAN_LEVEL(2)
+    TDSE_SCAN_LEVEL(4)
+    TDSE_SCAN_LEVEL(8)
+#undef TDSE_SCAN_LEVEL
+    // row totals: last lane of each row (forward) or first lane (reverse)
+    const int row = lane >> 4;
+    double2 tq[4], tz[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int src = FWD ? 16 * r + 15 : 16 * r;
+        tq[r] = readlane2(Q, src);
+        tz[r] = readlane2(z, src);
+    }
+    // carry into this row from the rows before it in scan order
+    double2 CQ = one, CZ = zero;
+    if (FWD) {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            if (r < row) {
+                double2 q = tq[r], zz = tz[r];
+                compose(q, zz, CQ, CZ);
+                CQ = q;
+                CZ = zz;
+            }
+    } else {
+#pragma unroll
+        for (int r = 3; r > 0; r--)
+            if (r > row) {
+                double2 q = tq[r], zz = tz[r];
+                compose(q, zz, CQ, CZ);
+                CQ = q;
+                CZ = zz;
+            }
+    }
+    compose(Q, z, CQ, CZ);
+}
+
 struct Acc {
     double pr, pi, pint, pos, gp;
 };
@@ -54,143 +136,156 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// Solve one segment of one step.  snap_row != nullptr: also store the new psi there (natural order).
-__device__ void solve_segment(const TdseGeom &G, int seg, int lane, const double2 *__restrict__ src,
-                              double2 *__restrict__ dst, double Fdt, double2 rot, Acc &acc, double *snap_row)
+// Cross-wave part of an exclusive scan of affine maps x_out = z + Q x_in.
+// (Qi, zi): the wave-inclusive composition of lanes [0..lane] (forward) or [lane..63]
+// (backward).  Returns the carry into this lane from everything before it in the
+// workgroup-wide lane order (zero at the start of the extended window).
+template <bool FWD>
+__device__ inline double2 block_carry(double2 Qi, double2 zi, int wave, int lane, int nw, double2 (*slot)[2])
+{
+    // exclusive within the wave: shift the inclusive maps by one lane
+    const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+    const double2 Qe = FWD ? dpp2<kWaveShr1>(Qi, one) : dpp2<kWaveShl1>(Qi, one);
+    const double2 ze = FWD ? dpp2<kWaveShr1>(zi, zero) : dpp2<kWaveShl1>(zi, zero);
+    if (lane == (FWD ? 63 : 0)) {
+        slot[wave][0] = Qi;
+        slot[wave][1] = zi;
+    }
+    __syncthreads();
+    // carry into this wave: compose the totals of the waves before it
+    double2 Z = make_double2(0.0, 0.0);
+    if (FWD) {
+        for (int v = 0; v < wave; v++) {
+            const double2 q = slot[v][0], z = slot[v][1];
+            Z = make_double2(z.x + q.x * Z.x - q.y * Z.y, z.y + q.x * Z.y + q.y * Z.x);
+        }
+    } else {
+        for (int v = nw - 1; v > wave; v--) {
+            const double2 q = slot[v][0], z = slot[v][1];
+            Z = make_double2(z.x + q.x * Z.x - q.y * Z.y, z.y + q.x * Z.y + q.y * Z.x);
+        }
+    }
+    return make_double2(ze.x + Qe.x * Z.x - Qe.y * Z.y, ze.y + Qe.x * Z.y + Qe.y * Z.x);
+}
+
+// Solve one segment of one step with all waves of the workgroup.
+// snap_row != nullptr: also store the new psi there (natural order).
+// Each lane keeps its chunk of psi and the chunk's coefficients (g_j, 1/beta_j) in
+// registers for the whole solve: all loads of a segment are issued together, once.
+// psi is updated in place: the segments of a step are solved in order, and the only
+// chunks a segment reads after the previous segment has overwritten them are its H
+// left-halo chunks, whose old values the previous segment saved to LDS (halo_old).
+__device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
+                              double2 *__restrict__ psi, double Fdt, double2 rot, Acc &acc, double *snap_row,
+                              double2 (*slot_f)[2], double2 (*slot_b)[2], double2 *halo_in, double2 *halo_out)
 {
     const int H = G.halo_lanes;
-    const int c = seg * G.own_lanes - H + lane;  // >= -H > -64: always inside the padded layout
-    const long long base = tdse_chunk_base(c);
-    const double *gt = G.g + base;
-    const double2 *ibt = G.ib + base;
-    // The table pointers are made opaque before every pass so that each pass reloads
-    // its coefficients from cache instead of keeping all of them live in registers.
-    auto fresh = [&]() {
-        asm volatile("" : "+v"(gt));
-        asm volatile("" : "+v"(ibt));
-    };
+    const int gl = wave * 64 + lane;                 // lane index in the workgroup
+    const int nl = nw * 64;
+    const int c = seg * own_chunks - H + gl;
+    // Lanes of the first and last segment reach past the grid; they read chunk -1 or
+    // chunk nchunks instead, which are zero in every buffer and table (padding rows).
+    const int cl = min(max(c, -1), G.nchunks);
+    const long long base = tdse_chunk_base(cl);
 
-    double2 P[C];
+    double2 P[C], IB[C];
+    double Gv[C];
+    const bool from_lds = seg > 0 && gl < H;               // chunk already overwritten by segment seg-1
+    const bool save_old = gl >= own_chunks && gl < own_chunks + H;  // next segment's left halo
 #pragma unroll
-    for (int i = 0; i < C; i++) P[i] = src[base + 64 * i];
-    // neighbours across the chunk edges; zero outside the extended window (truncation zone)
-    double2 left = shfl_up2(P[C - 1], 1), right = shfl_down2(P[0], 1);
-    const double g_first = gt[0], g_last = gt[64 * (C - 1)];
-    double gL = __shfl_up(g_last, 1, 64), gR = __shfl_down(g_first, 1, 64);
+    for (int i = 0; i < C; i++) {
+        P[i] = from_lds ? halo_in[gl * C + i] : psi[base + 64 * i];
+        Gv[i] = G.g[base + 64 * i];
+        IB[i] = G.ib[base + 64 * i];
+    }
+    if (save_old) {
+#pragma unroll
+        for (int i = 0; i < C; i++) halo_out[(gl - own_chunks) * C + i] = P[i];
+    }
+    const double2 Qc = G.chq[cl + 64];
+    const double2 Gc = G.chg[cl + 64];
+    // neighbours across the chunk edges: shuffles inside a wave, loads across waves,
+    // zero outside the extended window (truncation zone)
+    const double2 zero2 = make_double2(0.0, 0.0);
+    double2 left = dpp2<kWaveShr1>(P[C - 1], zero2), right = dpp2<kWaveShl1>(P[0], zero2);
+    double gL = dpp<kWaveShr1>(Gv[C - 1], 0.0), gR = dpp<kWaveShl1>(Gv[0], 0.0);
     if (lane == 0) {
-        left = make_double2(0.0, 0.0);
-        gL = 0.0;
+        const long long nb = tdse_chunk_base(min(max(c - 1, -1), G.nchunks)) + 64 * (C - 1);
+        left = gl > 0 ? psi[nb] : make_double2(0.0, 0.0);  // gl >= 64 > H: not yet overwritten
+        gL = gl > 0 ? G.g[nb] : 0.0;
     }
     if (lane == 63) {
-        right = make_double2(0.0, 0.0);
-        gR = 0.0;
+        const long long nb = tdse_chunk_base(min(max(c + 1, -1), G.nchunks));
+        right = gl < nl - 1 ? psi[nb] : make_double2(0.0, 0.0);
+        gR = gl < nl - 1 ? G.g[nb] : 0.0;
     }
 
     // r' = (M2' - i H') psi
     {
         double2 pm = left;
-        double gm = gL, g0 = g_first;
+        double gm = gL;
         const double K = G.k36;
 #pragma unroll
         for (int i = 0; i < C; i++) {
-            if (i % kWindow == 0) fresh();
             const double2 p0 = P[i];
             const double2 pp = (i + 1 < C) ? P[i + 1] : right;
-            const double gp = (i + 1 < C) ? gt[64 * (i + 1)] : gR;
-            const double d = fma(10.0, g0, K);
+            const double gp = (i + 1 < C) ? Gv[i + 1] : gR;
+            const double d = fma(10.0, Gv[i], K);
             const double mr = fma(10.0, p0.x, pm.x + pp.x);
             const double mi = fma(10.0, p0.y, pm.y + pp.y);
             const double hr = fma(gm, pm.x, fma(d, p0.x, gp * pp.x));
             const double hi = fma(gm, pm.y, fma(d, p0.y, gp * pp.y));
             P[i] = make_double2(mr + hi, mi - hr);
             pm = p0;
-            gm = g0;
-            g0 = gp;
+            gm = Gv[i];
         }
     }
 
     // forward recurrence z_j = ib_j (r_j - (1 + i g_{j-1}) z_{j-1})
     {
-        fresh();
         double2 z = make_double2(0.0, 0.0);
-        double gm = gL;
 #pragma unroll
         for (int i = 0; i < C; i++) {
-            if (i % kWindow == 0) fresh();
-            const double2 ib = ibt[64 * i];
+            const double gm = (i > 0) ? Gv[i - 1] : gL;
             const double tr = fma(gm, z.y, P[i].x - z.x);
             const double ti = fma(-gm, z.x, P[i].y - z.y);
-            z = cmul(make_double2(tr, ti), ib);
-            gm = gt[64 * i];
+            z = cmul(make_double2(tr, ti), IB[i]);
         }
-        // exclusive scan of the chunk maps z_out = z + Q z_in over the lanes
-        const double2 *qp = G.chq + (c + 64);
-        asm volatile("" : "+v"(qp));
-        double2 Q = *qp;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const double2 Qo = shfl_up2(Q, d), zo = shfl_up2(z, d);
-            if (lane >= d) {
-                z = make_double2(z.x + Q.x * zo.x - Q.y * zo.y, z.y + Q.x * zo.y + Q.y * zo.x);
-                Q = cmul(Q, Qo);
-            }
-        }
-        z = shfl_up2(z, 1);
-        if (lane == 0) z = make_double2(0.0, 0.0);
-        fresh();
-        gm = gL;
+        double2 Q = Qc;
+        wave_scan<true>(Q, z, lane);
+        z = block_carry<true>(Q, z, wave, lane, nw, slot_f);
 #pragma unroll
         for (int i = 0; i < C; i++) {
-            if (i % kWindow == 0) fresh();
-            const double2 ib = ibt[64 * i];
+            const double gm = (i > 0) ? Gv[i - 1] : gL;
             const double tr = fma(gm, z.y, P[i].x - z.x);
             const double ti = fma(-gm, z.x, P[i].y - z.y);
-            z = cmul(make_double2(tr, ti), ib);
+            z = cmul(make_double2(tr, ti), IB[i]);
             P[i] = z;
-            gm = gt[64 * i];
         }
     }
 
     // backward recurrence u_j = z_j + h_j u_{j+1},  h_j = -(1 + i g_{j+1}) ib_j
     {
-        fresh();
         double2 u = make_double2(0.0, 0.0);
-        double gp = gR;
 #pragma unroll
         for (int i = C - 1; i >= 0; i--) {
-            if ((C - 1 - i) % kWindow == 0) fresh();
-            const double2 ib = ibt[64 * i];
-            const double hr = fma(gp, ib.y, -ib.x), hi = -fma(gp, ib.x, ib.y);
+            const double gp = (i + 1 < C) ? Gv[i + 1] : gR;
+            const double hr = fma(gp, IB[i].y, -IB[i].x), hi = -fma(gp, IB[i].x, IB[i].y);
             u = make_double2(P[i].x + hr * u.x - hi * u.y, P[i].y + hr * u.y + hi * u.x);
-            gp = gt[64 * i];
         }
-        const double2 *gq = G.chg + (c + 64);
-        asm volatile("" : "+v"(gq));
-        double2 Gt = *gq;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const double2 Go = shfl_down2(Gt, d), uo = shfl_down2(u, d);
-            if (lane + d < 64) {
-                u = make_double2(u.x + Gt.x * uo.x - Gt.y * uo.y, u.y + Gt.x * uo.y + Gt.y * uo.x);
-                Gt = cmul(Gt, Go);
-            }
-        }
-        u = shfl_down2(u, 1);
-        if (lane == 63) u = make_double2(0.0, 0.0);
-        fresh();
-        gp = gR;
+        double2 Gt = Gc;
+        wave_scan<false>(Gt, u, lane);
+        u = block_carry<false>(Gt, u, wave, lane, nw, slot_b);
 #pragma unroll
         for (int i = C - 1; i >= 0; i--) {
-            if ((C - 1 - i) % kWindow == 0) fresh();
-            const double2 ib = ibt[64 * i];
-            const double hr = fma(gp, ib.y, -ib.x), hi = -fma(gp, ib.x, ib.y);
+            const double gp = (i + 1 < C) ? Gv[i + 1] : gR;
+            const double hr = fma(gp, IB[i].y, -IB[i].x), hi = -fma(gp, IB[i].x, IB[i].y);
             u = make_double2(P[i].x + hr * u.x - hi * u.y, P[i].y + hr * u.y + hi * u.x);
             P[i] = u;
-            gp = gt[64 * i];
         }
     }
 
-    if (lane < H || lane >= 64 - H || c >= G.nchunks) return;  // only the segment's own chunks continue
+    if (gl < H || gl >= nl - H || c >= G.nchunks) return;  // only the segment's own chunks continue
 
     // length-gauge phase exp(+i F dt x_j) (prop.c:199-204) by rotation from the chunk start,
     // store, and the observables of prop.c:302-333 that need every point
@@ -202,10 +297,9 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int lane, const double
     const double *dp = G.dpot + base;
 #pragma unroll
     for (int i = 0; i < C; i++) {
-        if (i % kWindow == 0) asm volatile("" : "+v"(dp));
         const double2 v = cmul(e, P[i]);
         P[i] = v;
-        dst[base + 64 * i] = v;
+        psi[base + 64 * i] = v;
         const double n2 = fma(v.x, v.x, v.y * v.y);
         const double xj = __dsub_rn(__dmul_rn((double)(j0 + i), G.dx), G.xmax);
         acc.pos = fma(n2, xj, acc.pos);
@@ -217,7 +311,6 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int lane, const double
         const double2 *q0 = G.psi0 + base;
 #pragma unroll
         for (int i = 0; i < C; i++) {
-            if (i % kWindow == 0) asm volatile("" : "+v"(q0));
             const double2 q = q0[64 * i];
             acc.pr = fma(P[i].x, q.x, fma(P[i].y, q.y, acc.pr));
             acc.pi = fma(P[i].x, q.y, fma(-P[i].y, q.x, acc.pi));
@@ -244,17 +337,18 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int lane, const double
 
 } // namespace
 
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(64 * TDSE_MAX_WAVES)
 tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst, int *counter,
                       double2 *work, long long work_stride)
 {
     __shared__ double part[2][TDSE_MAX_WAVES][5];
+    __shared__ double2 slot_f[TDSE_MAX_WAVES][2], slot_b[TDSE_MAX_WAVES][2];
+    __shared__ double2 halo[2][TDSE_MAX_HALO * TDSE_CHUNK];
     __shared__ int cur;
     const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    double2 *bufA = work + (long long)blockIdx.x * 2 * work_stride;
-    double2 *bufB = bufA + work_stride;
+    double2 *buf = work + (long long)blockIdx.x * work_stride;
 
     for (;;) {
         if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
@@ -265,9 +359,12 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
         const TdseInst &I = insts[order[t]];
         const TdseGeom &G = geoms[I.geom];
         const long long npad = G.npad;
+        const int own_chunks = nw * 64 - 2 * G.halo_lanes;  // host guarantees > 0
+        const int nseg = (G.nchunks + own_chunks - 1) / own_chunks;
 
-        // psi(t = tmin) = psi0
-        for (long long q = threadIdx.x; q < npad; q += blockDim.x) bufA[q] = G.psi0[q];
+        // psi(t = tmin) = psi0 in both buffers: the padding chunks (zero rows outside the grid)
+        // are read as halo but never written by the steps, so they must start at zero in each
+        for (long long q = threadIdx.x; q < npad; q += blockDim.x) buf[q] = G.psi0[q];
         if (threadIdx.x == 0) {
             I.src[0] = 0.0;
             I.ptot[0] = 1.0;
@@ -298,10 +395,10 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
                 snap_row = I.snap + (long long)i_snap * 2 * G.nx;
                 i_snap++;
             }
-            const double2 *src = (k & 1) ? bufB : bufA;
-            double2 *dst = (k & 1) ? bufA : bufB;
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int s = wave; s < G.nseg; s += nw) solve_segment(G, s, lane, src, dst, Fdt, rot, acc, snap_row);
+            for (int s = 0; s < nseg; s++)
+                solve_segment(G, s, own_chunks, wave, lane, nw, buf, Fdt, rot, acc, snap_row, slot_f, slot_b,
+                              halo[s & 1], halo[(s + 1) & 1]);
             // a final snapshot that coincides with the last step is written after the loop
             const double pr = wave_sum(acc.pr), pi = wave_sum(acc.pi), pint = wave_sum(acc.pint);
             const double pos = wave_sum(acc.pos), gp = wave_sum(acc.gp);
@@ -333,7 +430,7 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
         // earlier row was used; the final state itself is optional output
         const bool last_snap = I.snap && I.n_snap > 0 && i_snap == I.n_snap - 1;
         if (last_snap || I.psi_final) {
-            const double2 *fin = (I.n_steps & 1) ? bufB : bufA;
+            const double2 *fin = buf;
             double *row = last_snap ? I.snap + (long long)(I.n_snap - 1) * 2 * G.nx : nullptr;
             for (int j = threadIdx.x; j < G.nx; j += blockDim.x) {
                 const double2 v = fin[tdse_index(j)];

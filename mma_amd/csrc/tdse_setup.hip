@@ -385,8 +385,9 @@ __global__ void tdse_spectrum_kernel(const double2 *X, int nbins, double dt, dou
 }
 
 // FourInterp's inverse half (tools_fftw3.c:46-63) for k1 >= 2: bins holds the first
-// nbins = floor(n_in/2)+1 bins of the input spectrum already divided by n_in; they sit
-// below the Nyquist bin of length n_out, so each counts twice (unnormalised c2r).
+// nbins = floor(n_in/2)+1 bins of the input's DFT; each is divided by n_in (the
+// reference's normalisation) and, sitting below the Nyquist bin of length n_out,
+// counts twice in the unnormalised c2r.
 __global__ void tdse_c2r_interp_kernel(const double2 *bins, int nbins, int n_in, int n_out, double *out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -407,6 +408,5 @@ __global__ void tdse_c2r_interp_kernel(const double2 *bins, int nbins, int n_in,
             c = cn;
         }
     }
-    out[t] = bins[0].x + 2.0 * acc;
-    (void)n_in;
+    out[t] = (bins[0].x + 2.0 * acc) / (double)n_in;
 }
