@@ -197,6 +197,36 @@ void ctdse_geometry_destroy(ctdse_geometry *g);
 int ctdse_propagate_batch(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const double *field,
                           const long long *field_off, double *out, const long long *out_off, void *stream);
 
+/* Length, in complex doubles, of one device state vector of this geometry (the tiled
+ * layout with its zero padding); states below are arrays of such vectors. */
+long long ctdse_state_length(const ctdse_geometry *g);
+
+/*
+ * Partial forward runs for checkpointed gradients (no counterpart in the reference).
+ * Instance i runs steps k0[i] .. k0[i]+n_run[i]-1 (n_run = 0: k0 .. n_steps-1) of a run of
+ * n_steps[i] steps, driven by the already gated field F (field + field_off[i], indexed by
+ * step), starting from psi_init + init_off[i] (a state vector; psi_init NULL: the ground
+ * state, then k0 must be 0).  After every save_every-th step the state is written to
+ * save + save_off[i] + m * state_length (m = 0, 1, ...; save may be NULL; the buffer must
+ * start zeroed: only grid rows are written).  out (may be NULL) receives the observables as
+ * in ctdse_propagate_batch.  All arrays are device memory; offsets are host arrays in doubles.
+ */
+int ctdse_propagate_states(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const int *k0,
+                           const int *n_run, const double *field, const long long *field_off, const double *psi_init,
+                           const long long *init_off, double *save, const long long *save_off, int save_every,
+                           double *out, const long long *out_off, void *stream);
+
+/*
+ * Adjoint sweep (no counterpart in the reference): for k = k_hi[i] down to k_lo[i]+1 it
+ * updates the adjoint state kappa (a state vector, start from zero at k = n_steps-1) and
+ * writes grad[k-1] = dL/dF_{k-1}, given gobs[k] = dL/d sourceterm_k, the gated field F and
+ * the states psi_k at states + states_off[i] + (k - k_lo[i] - 1) * state_length.
+ */
+int ctdse_adjoint_batch(int n_inst, const ctdse_geometry *const *geom, const int *k_hi, const int *k_lo,
+                        const double *field, const long long *field_off, const double *gobs, const long long *gobs_off,
+                        const double *states, const long long *states_off, double *kappa, const long long *kappa_off,
+                        double *grad, const long long *grad_off, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,7 @@ SOURCES = [
     ("tdse_setup.hip", ["-ffp-contract=off"]),
     ("tdse_analysis.hip", ["-ffp-contract=off"]),
     ("tdse_propagate.hip", []),
+    ("tdse_adjoint.hip", []),
     ("ctdse_abi.cpp", ["-x", "hip"]),
 ]
 

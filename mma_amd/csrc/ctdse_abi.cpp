@@ -83,7 +83,7 @@ struct ctdse_geometry {
     double dx = 0, dt = 0, a = 0, x_int = 0;
     double e_init = 0;
     int gs_iters = 0;
-    DevMem x, psi0_nat, g, dpot, ib, psi0, chq, chg;
+    DevMem x, psi0_nat, g, dpot, ib, psi0, chq, chg, adq, adg;
 };
 
 namespace {
@@ -131,6 +131,8 @@ int build_tables(ctdse_geometry *G)
     HIP_CHECK(G->psi0.alloc(npad * sizeof(double2)));
     HIP_CHECK(G->chq.alloc((nchunks + 128) * sizeof(double2)));
     HIP_CHECK(G->chg.alloc((nchunks + 128) * sizeof(double2)));
+    HIP_CHECK(G->adq.alloc((nchunks + 128) * sizeof(double2)));
+    HIP_CHECK(G->adg.alloc((nchunks + 128) * sizeof(double2)));
     HIP_CHECK(piv.alloc(2 * (size_t)nx * sizeof(double)));
     HIP_CHECK(job.alloc(sizeof(TableJob)));
     TableJob J{};
@@ -147,6 +149,8 @@ int build_tables(ctdse_geometry *G)
     J.psi0 = G->psi0.as<double2>();
     J.chq = G->chq.as<double2>();
     J.chg = G->chg.as<double2>();
+    J.adq = G->adq.as<double2>();
+    J.adg = G->adg.as<double2>();
     J.pivot_work = piv.as<double>();
     HIP_CHECK(hipMemcpy(job.p, &J, sizeof J, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(tdse_tables_grid_kernel, dim3(256), dim3(256), 0, 0, job.as<TableJob>());
@@ -173,6 +177,8 @@ int build_tables(ctdse_geometry *G)
     D.psi0 = G->psi0.as<double2>();
     D.chq = G->chq.as<double2>();
     D.chg = G->chg.as<double2>();
+    D.adq = G->adq.as<double2>();
+    D.adg = G->adg.as<double2>();
     D.nx = nx;
     D.nchunks = nchunks;
     D.npad = npad;
@@ -237,11 +243,47 @@ struct InstSpec {
     const ctdse_geometry *geom;
     int n_steps;
     const double *field;     // device
-    double *out;             // device, 5 rows of n_steps
+    double *out;             // device, 5 rows of n_steps (may be null for partial runs)
     double *snap = nullptr;  // device
     int snap_every = 1, n_snap = 0;
     double *psi_final = nullptr;
+    int k0 = 0, n_run = 0, pregated = 0;
+    const double2 *psi_init = nullptr;
+    double2 *save = nullptr;
+    int save_every = 1;
 };
+
+// waves per workgroup for a batch: a segment is nw*64 chunks of which 2H are truncation
+// halo; pick the nw that wastes least on the work-weighted geometries, mildly preferring
+// occupancy (TDSE_WAVES_PER_CU waves fit on a CU)
+int pick_waves(const std::vector<const ctdse_geometry *> &geoms, const std::vector<double> &work,
+               const std::vector<const ctdse_geometry *> &inst_geom, int *min_out)
+{
+    int min_nw = 1;
+    for (const ctdse_geometry *g : geoms)
+        while (min_nw * 64 - 2 * g->dev.halo_lanes < 64) min_nw++;
+    int nw = std::max(4, min_nw);
+    double best = -1.0;
+    for (int w = std::max(4, min_nw); w <= TDSE_MAX_WAVES; w++) {
+        double used = 0.0, lanes = 0.0;
+        for (size_t i = 0; i < inst_geom.size(); i++) {
+            const TdseGeom &d = inst_geom[i]->dev;
+            const int own = w * 64 - 2 * d.halo_lanes;
+            const int nseg = (d.nchunks + own - 1) / own;
+            used += (double)d.nchunks * work[i];
+            lanes += (double)nseg * w * 64 * work[i];
+        }
+        const double occ = (double)((TDSE_WAVES_PER_CU / w) * w) / TDSE_WAVES_PER_CU;
+        const double score = (used / lanes) * std::pow(occ, 0.3);
+        if (score > best) {
+            best = score;
+            nw = w;
+        }
+    }
+    if (const char *env = std::getenv("MMA_TDSE_WAVES")) nw = std::max(min_nw, std::min(TDSE_MAX_WAVES, std::atoi(env)));
+    *min_out = min_nw;
+    return nw;
+}
 
 int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
 {
@@ -252,7 +294,8 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
     long long stride = 0;
     for (int i = 0; i < n; i++) {
         const InstSpec &s = spec[i];
-        if (!s.geom || s.n_steps < 1 || !s.field || !s.out)
+        if (!s.geom || s.n_steps < 1 || !s.field || (!s.out && !s.save && s.n_run == 0) || s.k0 < 0 || s.n_run < 0 ||
+            s.k0 + s.n_run > s.n_steps || (s.k0 > 0 && !s.pregated) || (s.save && s.save_every < 1))
             return set_status(CTDSE_ERR_ARGUMENT, "instance " + std::to_string(i) + ": bad descriptor");
         int gi = (int)(std::find(geoms.begin(), geoms.end(), s.geom) - geoms.begin());
         if (gi == (int)geoms.size()) geoms.push_back(s.geom);
@@ -261,10 +304,16 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
         I.n_steps = s.n_steps;
         I.field = s.field;
         I.src = s.out;
-        I.ptot = s.out + s.n_steps;
-        I.pint = s.out + 2LL * s.n_steps;
-        I.expv = s.out + 3LL * s.n_steps;
-        I.efield = s.out + 4LL * s.n_steps;
+        I.ptot = s.out ? s.out + s.n_steps : nullptr;
+        I.pint = s.out ? s.out + 2LL * s.n_steps : nullptr;
+        I.expv = s.out ? s.out + 3LL * s.n_steps : nullptr;
+        I.efield = s.out ? s.out + 4LL * s.n_steps : nullptr;
+        I.k0 = s.k0;
+        I.n_run = s.n_run;
+        I.pregated = s.pregated;
+        I.psi_init = s.psi_init;
+        I.save = s.save;
+        I.save_every = s.save_every;
         I.snap = s.snap;
         I.snap_every = std::max(1, s.snap_every);
         I.n_snap = s.n_snap;
@@ -274,8 +323,9 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
     // largest instances first (longest-processing-time order for the atomic queue)
     std::vector<int> order(n);
     for (int i = 0; i < n; i++) order[i] = i;
+    auto steps_of = [&](int p) { return (double)(spec[p].n_run > 0 ? spec[p].n_run : spec[p].n_steps - spec[p].k0); };
     std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
-        return (double)spec[p].geom->dev.nx * spec[p].n_steps > (double)spec[q].geom->dev.nx * spec[q].n_steps;
+        return (double)spec[p].geom->dev.nx * steps_of(p) > (double)spec[q].geom->dev.nx * steps_of(q);
     });
     std::vector<TdseGeom> gdev(geoms.size());
     for (size_t k = 0; k < geoms.size(); k++) gdev[k] = geoms[k]->dev;
@@ -284,31 +334,14 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
     HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    // Waves per workgroup: a segment is nw*64 chunks of which 2H are truncation halo; pick the
-    // nw that wastes least on the work-weighted geometries, mildly preferring occupancy
-    // (TDSE_WAVES_PER_CU waves fit on a CU).
-    int min_nw = 1;
-    for (const ctdse_geometry *g : geoms)
-        while (min_nw * 64 - 2 * g->dev.halo_lanes < 64) min_nw++;
-    int nw = std::max(4, min_nw);
-    double best = -1.0;
-    for (int w = std::max(4, min_nw); w <= TDSE_MAX_WAVES; w++) {
-        double used = 0.0, lanes = 0.0;
-        for (int i = 0; i < n; i++) {
-            const TdseGeom &d = spec[i].geom->dev;
-            const int own = w * 64 - 2 * d.halo_lanes;
-            const int nseg = (d.nchunks + own - 1) / own;
-            used += (double)d.nchunks * spec[i].n_steps;
-            lanes += (double)nseg * w * 64 * spec[i].n_steps;
-        }
-        const double occ = (double)((TDSE_WAVES_PER_CU / w) * w) / TDSE_WAVES_PER_CU;
-        const double score = (used / lanes) * std::pow(occ, 0.3);
-        if (score > best) {
-            best = score;
-            nw = w;
-        }
+    std::vector<double> work(n);
+    std::vector<const ctdse_geometry *> inst_geom(n);
+    for (int i = 0; i < n; i++) {
+        work[i] = steps_of(i);
+        inst_geom[i] = spec[i].geom;
     }
-    if (const char *env = std::getenv("MMA_TDSE_WAVES")) nw = std::max(min_nw, std::min(TDSE_MAX_WAVES, std::atoi(env)));
+    int min_nw = 1;
+    const int nw = pick_waves(geoms, work, inst_geom, &min_nw);
     int per_cu = std::max(1, TDSE_WAVES_PER_CU / nw);
     if (const char *env = std::getenv("MMA_TDSE_WG_PER_CU")) per_cu = std::max(1, std::atoi(env));
     const int slots = std::max(1, std::min(n, prop.multiProcessorCount * per_cu));
@@ -330,6 +363,51 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
     hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, d_counter.as<int>(), 1, 0);
     hipLaunchKernelGGL(tdse_propagate_kernel, dim3(slots), dim3(64 * nw), 0, stream, d_geom.as<TdseGeom>(),
                        d_inst.as<TdseInst>(), d_order.as<int>(), n, d_counter.as<int>(), g_work.as<double2>(), stride);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return CTDSE_OK;
+}
+
+int adjoint(const std::vector<const ctdse_geometry *> &ig, const std::vector<TdseAdjInst> &ai, hipStream_t stream)
+{
+    const int n = (int)ai.size();
+    if (n == 0) return CTDSE_OK;
+    std::vector<const ctdse_geometry *> geoms;
+    std::vector<TdseAdjInst> insts = ai;
+    std::vector<double> work(n);
+    for (int i = 0; i < n; i++) {
+        int gi = (int)(std::find(geoms.begin(), geoms.end(), ig[i]) - geoms.begin());
+        if (gi == (int)geoms.size()) geoms.push_back(ig[i]);
+        insts[i].geom = gi;
+        work[i] = (double)(ai[i].k_hi - ai[i].k_lo);
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+        return (double)ig[p]->dev.nx * work[p] > (double)ig[q]->dev.nx * work[q];
+    });
+    std::vector<TdseGeom> gdev(geoms.size());
+    for (size_t k = 0; k < geoms.size(); k++) gdev[k] = geoms[k]->dev;
+    int min_nw = 1;
+    const int nw = pick_waves(geoms, work, ig, &min_nw);
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    const int per_cu = std::max(1, TDSE_WAVES_PER_CU / nw);
+    const int slots = std::max(1, std::min(n, prop.multiProcessorCount * per_cu));
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    DevMem d_geom, d_inst, d_order, d_counter;
+    HIP_CHECK(d_geom.alloc(gdev.size() * sizeof(TdseGeom)));
+    HIP_CHECK(d_inst.alloc(insts.size() * sizeof(TdseAdjInst)));
+    HIP_CHECK(d_order.alloc(order.size() * sizeof(int)));
+    HIP_CHECK(d_counter.alloc(sizeof(int)));
+    HIP_CHECK(hipMemcpyAsync(d_geom.p, gdev.data(), gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_inst.p, insts.data(), insts.size() * sizeof(TdseAdjInst), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, d_counter.as<int>(), 1, 0);
+    hipLaunchKernelGGL(tdse_adjoint_kernel, dim3(slots), dim3(64 * nw), 0, stream, d_geom.as<TdseGeom>(),
+                       d_inst.as<TdseAdjInst>(), d_order.as<int>(), n, d_counter.as<int>());
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(stream));
     return CTDSE_OK;
@@ -420,6 +498,70 @@ int ctdse_propagate_batch(int n_inst, const ctdse_geometry *const *geom, const i
         spec[i].out = out + out_off[i];
     }
     return propagate(spec, (hipStream_t)stream);
+}
+
+long long ctdse_state_length(const ctdse_geometry *g)
+{
+    clear_status();
+    if (!g) {
+        set_status(CTDSE_ERR_ARGUMENT, "null geometry");
+        return -1;
+    }
+    return g->dev.npad;
+}
+
+int ctdse_propagate_states(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const int *k0,
+                           const int *n_run, const double *field, const long long *field_off, const double *psi_init,
+                           const long long *init_off, double *save, const long long *save_off, int save_every,
+                           double *out, const long long *out_off, void *stream)
+{
+    clear_status();
+    if (n_inst < 0 || (n_inst > 0 && (!geom || !n_steps || !k0 || !n_run || !field || !field_off)))
+        return set_status(CTDSE_ERR_ARGUMENT, "null batch argument");
+    if (device_ready() != CTDSE_OK) return g_status;
+    std::vector<InstSpec> spec(n_inst);
+    for (int i = 0; i < n_inst; i++) {
+        InstSpec &t = spec[i];
+        t.geom = geom[i];
+        t.n_steps = n_steps[i];
+        t.k0 = k0[i];
+        t.n_run = n_run[i];
+        t.pregated = 1;
+        t.field = field + field_off[i];
+        t.psi_init = psi_init ? (const double2 *)(psi_init + init_off[i]) : nullptr;
+        t.save = save ? (double2 *)(save + save_off[i]) : nullptr;
+        t.save_every = save_every;
+        t.out = out ? out + out_off[i] : nullptr;
+    }
+    return propagate(spec, (hipStream_t)stream);
+}
+
+int ctdse_adjoint_batch(int n_inst, const ctdse_geometry *const *geom, const int *k_hi, const int *k_lo,
+                        const double *field, const long long *field_off, const double *gobs, const long long *gobs_off,
+                        const double *states, const long long *states_off, double *kappa, const long long *kappa_off,
+                        double *grad, const long long *grad_off, void *stream)
+{
+    clear_status();
+    if (n_inst < 0 || (n_inst > 0 && (!geom || !k_hi || !k_lo || !field || !field_off || !gobs || !gobs_off ||
+                                      !states || !states_off || !kappa || !kappa_off || !grad || !grad_off)))
+        return set_status(CTDSE_ERR_ARGUMENT, "null adjoint argument");
+    if (device_ready() != CTDSE_OK) return g_status;
+    std::vector<const ctdse_geometry *> ig(n_inst);
+    std::vector<TdseAdjInst> ai(n_inst);
+    for (int i = 0; i < n_inst; i++) {
+        if (!geom[i] || k_hi[i] <= k_lo[i] || k_lo[i] < 0)
+            return set_status(CTDSE_ERR_ARGUMENT, "adjoint instance " + std::to_string(i) + ": bad range");
+        ig[i] = geom[i];
+        TdseAdjInst &a = ai[i];
+        a.k_hi = k_hi[i];
+        a.k_lo = k_lo[i];
+        a.field = field + field_off[i];
+        a.gobs = gobs + gobs_off[i];
+        a.psi = (const double2 *)(states + states_off[i]);
+        a.kappa = (double2 *)(kappa + kappa_off[i]);
+        a.grad = grad + grad_off[i];
+    }
+    return adjoint(ig, ai, (hipStream_t)stream);
 }
 
 void Initialise_grid_and_ground_state(inputs_def *in)

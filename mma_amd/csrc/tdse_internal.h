@@ -45,6 +45,9 @@ struct TdseGeom {
     const double2 *psi0;  // ground state, tiled
     const double2 *chq;   // per chunk c at [c + 64]: product of q_j = -a'_j ib_j (forward transfer)
     const double2 *chg;   // per chunk c at [c + 64]: product of h_j = -c'_j ib_j (backward transfer)
+    const double2 *adq;   // per chunk c at [c + 64]: product of conj(h_{j-1})  (adjoint forward transfer)
+    const double2 *adg;   // per chunk c at [c + 64]: product of conj(p_j), p_j = -(1 + i g_j) ib_j
+                          //                           (adjoint backward transfer)
     int nx;               // num_r + 1 grid points
     int nchunks;          // ceil(nx / TDSE_CHUNK)
     long long npad;       // tiled array length, tdse_tiled_size(nchunks)
@@ -61,11 +64,28 @@ struct TdseInst {
     int geom;             // index into the geometry array
     int n_steps;          // outputs.Nt of the reference: steps run and samples exported
     const double *field;  // in_field, n_steps + 1 samples
-    double *src, *ptot, *pint, *expv;  // n_steps samples each
+    double *src, *ptot, *pint, *expv;  // n_steps samples each (src == null: no observables)
     double *efield;       // gated field, n_steps samples (may be null)
     double *snap;         // snapshots (natural order, interleaved), n_snap rows (may be null)
     int snap_every, n_snap;
     double *psi_final;    // final state, natural order (may be null)
+    // partial runs (checkpointed gradients): steps k0 .. k0+n_run-1 starting from psi_init
+    int k0, n_run;        // n_run = 0: the whole run 0 .. n_steps-1
+    int pregated;         // field holds the gated F_k already (no gating test)
+    const double2 *psi_init;  // tiled start state (null: psi0)
+    double2 *save;        // tiled states saved after every save_every-th step (may be null)
+    int save_every;
+};
+
+// One instance of an adjoint (backward) sweep over steps k_hi .. k_lo+1, see tdse_adjoint_kernel.
+struct TdseAdjInst {
+    int geom;
+    int k_hi, k_lo;           // produces kappa_{k-1} and dL/dF_{k-1} for k = k_hi down to k_lo + 1
+    const double *field;      // gated field F_k
+    const double *gobs;       // dL/d sourceterm_k
+    const double2 *psi;       // tiled states psi_k for k = k_lo+1 .. k_hi at psi + (k - k_lo - 1) * npad
+    double2 *kappa;           // tiled adjoint state, updated in place
+    double *grad;             // dL/dF_k, written for k = k_hi-1 down to k_lo
 };
 
 // Ground-state job: inverse iteration in one lane (bit-identical to the CPU reference).
@@ -86,7 +106,7 @@ struct TableJob {
     const double *x;        // natural order grid (from the ground-state job or the caller)
     const double *psi0_nat; // natural order interleaved ground state
     double *g, *dpot;       // tiled
-    double2 *ib, *psi0, *chq, *chg;  // chq/chg: nchunks + 128 entries, chunk c at c + 64
+    double2 *ib, *psi0, *chq, *chg, *adq, *adg;  // per-chunk tables: nchunks + 128 entries, chunk c at c + 64
     double *pivot_work;     // 2 (num_r + 1) scratch (natural-order 1/beta)
     double rho_fwd, rho_bwd, pint0;
     int pint_lo, pint_hi, p0_lo, p0_hi;
@@ -99,6 +119,8 @@ __global__ void tdse_tables_pivot_kernel(TableJob *job);
 __global__ void tdse_tables_chunk_kernel(TableJob *job);
 __global__ void tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
                                       int *counter, double2 *work, long long work_stride);
+__global__ void tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *insts, const int *order, int n_inst,
+                                    int *counter);
 __global__ void tdse_fill_int_kernel(int *p, int n, int v);
 __global__ void tdse_xgrid_kernel(double *x, int num_r, double dx);
 __global__ void tdse_tgrid_kernel(double *tgrid, int n, double tmin, double dt);

@@ -200,6 +200,8 @@ __global__ void tdse_tables_grid_kernel(TableJob *job)
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nchunks + 128; c += (long long)gridDim.x * blockDim.x) {
         J.chq[c] = make_double2(0., 0.);
         J.chg[c] = make_double2(0., 0.);
+        J.adq[c] = make_double2(0., 0.);
+        J.adg[c] = make_double2(0., 0.);
     }
 }
 
@@ -239,9 +241,10 @@ __global__ void tdse_tables_pivot_kernel(TableJob *job)
             const double gm = gval(j - 1);
             rho_f = fmax(rho_f, __builtin_sqrt(1.0 + gm * gm) * inv_abs);
         }
-        if (j < nx - 1) {  // |h_j| = |c'_j| |ib_j|
-            const double gp = gval(j + 1);
+        if (j < nx - 1) {  // |h_j| = |c'_j| |ib_j|, and the adjoint's |p_j| = |1 + i g_j| |ib_j|
+            const double gp = gval(j + 1), g0 = gval(j);
             rho_b = fmax(rho_b, __builtin_sqrt(1.0 + gp * gp) * inv_abs);
+            rho_b = fmax(rho_b, __builtin_sqrt(1.0 + g0 * g0) * inv_abs);
         }
         J.ib[tdse_index(j)] = make_double2(ir, ii);
     }
@@ -287,27 +290,27 @@ __global__ void tdse_tables_chunk_kernel(TableJob *job)
     const double gk = (-3.0 * J.dt) / (J.dx * J.dx);
     auto gval = [&](int j) { return (j < 0 || j >= nx) ? 0.0 : gk + (0.5 * J.dt) * soft_coulomb(J.x[j], J.a); };
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
-        double qr = 1.0, qi = 0.0, hr = 1.0, hi = 0.0;
+        auto ib_at = [&](int j) {
+            return (j >= 0 && j < nx) ? make_double2(J.pivot_work[2 * j], J.pivot_work[2 * j + 1]) : make_double2(0., 0.);
+        };
+        auto mul = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+        // -(1 + i g) ib
+        auto coef = [](double g, double2 ib) { return make_double2(-(ib.x - g * ib.y), -(ib.y + g * ib.x)); };
+        double2 Q = make_double2(1., 0.), Hh = make_double2(1., 0.), AQ = make_double2(1., 0.), AG = make_double2(1., 0.);
         for (int i = 0; i < TDSE_CHUNK; i++) {
             const int j = c * TDSE_CHUNK + i;
-            double ibr = 0.0, ibi = 0.0;
-            if (j < nx) {
-                ibr = J.pivot_work[2 * j];
-                ibi = J.pivot_work[2 * j + 1];
-            }
-            const double gm = gval(j - 1), gp = gval(j + 1);
-            // q_j = -(1 + i g_{j-1}) ib_j ;  h_j = -(1 + i g_{j+1}) ib_j
-            const double q_r = -(ibr - gm * ibi), q_i = -(ibi + gm * ibr);
-            const double h_r = -(ibr - gp * ibi), h_i = -(ibi + gp * ibr);
-            double t = qr * q_r - qi * q_i;
-            qi = qr * q_i + qi * q_r;
-            qr = t;
-            t = hr * h_r - hi * h_i;
-            hi = hr * h_i + hi * h_r;
-            hr = t;
+            const double2 ib = ib_at(j);
+            Q = mul(Q, coef(gval(j - 1), ib));       // q_j
+            Hh = mul(Hh, coef(gval(j + 1), ib));     // h_j
+            const double2 hm = coef(gval(j), ib_at(j - 1));  // h_{j-1} = -(1 + i g_j) ib_{j-1}
+            AQ = mul(AQ, make_double2(hm.x, -hm.y));
+            const double2 pj = coef(gval(j), ib);   // p_j = -(1 + i g_j) ib_j
+            AG = mul(AG, make_double2(pj.x, -pj.y));
         }
-        J.chq[c + 64] = make_double2(qr, qi);
-        J.chg[c + 64] = make_double2(hr, hi);
+        J.chq[c + 64] = Q;
+        J.chg[c + 64] = Hh;
+        J.adq[c + 64] = AQ;
+        J.adg[c + 64] = AG;
     }
 }
 

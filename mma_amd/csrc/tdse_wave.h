@@ -1,0 +1,151 @@
+// Wave- and workgroup-level building blocks of the segmented Crank-Nicolson solves
+// (forward propagation and its adjoint): cross-lane moves in the VALU (DPP), inclusive
+// scans of complex affine maps over a wave, and the carry across the waves of a
+// workgroup through LDS.
+#pragma once
+#include "tdse_internal.h"
+
+namespace tdse {
+
+constexpr int C = TDSE_CHUNK;
+
+__device__ inline double2 shfl_up2(double2 v, int d)
+{
+    return make_double2(__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64));
+}
+__device__ inline double2 shfl_down2(double2 v, int d)
+{
+    return make_double2(__shfl_down(v.x, d, 64), __shfl_down(v.y, d, 64));
+}
+__device__ inline double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// ---- cross-lane moves in the VALU (DPP), 64-bit values as two 32-bit halves ----
+// Lanes without a source keep `old` (bound_ctrl off), so scans pad with the identity map.
+constexpr int kRowShr = 0x110, kRowShl = 0x100, kWaveShr1 = 0x138, kWaveShl1 = 0x130;
+
+template <int CTRL>
+__device__ inline double dpp(double v, double old)
+{
+    const long long b = __builtin_bit_cast(long long, v), o = __builtin_bit_cast(long long, old);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL>
+__device__ inline double2 dpp2(double2 v, double2 old)
+{
+    return make_double2(dpp<CTRL>(v.x, old.x), dpp<CTRL>(v.y, old.y));
+}
+__device__ inline double readlane(double v, int l)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double2 readlane2(double2 v, int l) { return make_double2(readlane(v.x, l), readlane(v.y, l)); }
+
+// (Q, z) <- (Q, z) o (Qo, zo): apply the map (Qo, zo) first, then (Q, z)
+__device__ inline void compose(double2 &Q, double2 &z, double2 Qo, double2 zo)
+{
+    z = make_double2(z.x + Q.x * zo.x - Q.y * zo.y, z.y + Q.x * zo.y + Q.y * zo.x);
+    Q = make_double2(Q.x * Qo.x - Q.y * Qo.y, Q.x * Qo.y + Q.y * Qo.x);
+}
+
+// Inclusive scan over the 64 lanes of the affine maps x_out = z + Q x_in, in lane order
+// (FWD: lane l composes lanes 0..l) or reverse order (lanes l..63).  Rows of 16 lanes
+// scan with DPP shifts; rows are then joined through the row totals read with readlane.
+template <bool FWD>
+__device__ inline void wave_scan(double2 &Q, double2 &z, int lane)
+{
+    const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+#define TDSE_SCAN_LEVEL(D)                                                                                   \
+    {                                                                                                        \
+        const double2 Qo = FWD ? dpp2<kRowShr + D>(Q, one) : dpp2<kRowShl + D>(Q, one);                      \
+        const double2 zo = FWD ? dpp2<kRowShr + D>(z, zero) : dpp2<kRowShl + D>(z, zero);                    \
+        compose(Q, z, Qo, zo);                                                                               \
+    }
+    TDSE_SCAN_LEVEL(1)
+    TDSE_SCAN_LEVEL(2)
+    TDSE_SCAN_LEVEL(4)
+    TDSE_SCAN_LEVEL(8)
+#undef TDSE_SCAN_LEVEL
+    // row totals: last lane of each row (forward) or first lane (reverse)
+    const int row = lane >> 4;
+    double2 tq[4], tz[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int src = FWD ? 16 * r + 15 : 16 * r;
+        tq[r] = readlane2(Q, src);
+        tz[r] = readlane2(z, src);
+    }
+    // carry into this row from the rows before it in scan order
+    double2 CQ = one, CZ = zero;
+    if (FWD) {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            if (r < row) {
+                double2 q = tq[r], zz = tz[r];
+                compose(q, zz, CQ, CZ);
+                CQ = q;
+                CZ = zz;
+            }
+    } else {
+#pragma unroll
+        for (int r = 3; r > 0; r--)
+            if (r > row) {
+                double2 q = tq[r], zz = tz[r];
+                compose(q, zz, CQ, CZ);
+                CQ = q;
+                CZ = zz;
+            }
+    }
+    compose(Q, z, CQ, CZ);
+}
+
+struct Acc {
+    double pr, pi, pint, pos, gp;
+};
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Cross-wave part of an exclusive scan of affine maps x_out = z + Q x_in.
+// (Qi, zi): the wave-inclusive composition of lanes [0..lane] (forward) or [lane..63]
+// (backward).  Returns the carry into this lane from everything before it in the
+// workgroup-wide lane order (zero at the start of the extended window).
+template <bool FWD>
+__device__ inline double2 block_carry(double2 Qi, double2 zi, int wave, int lane, int nw, double2 (*slot)[2])
+{
+    // exclusive within the wave: shift the inclusive maps by one lane
+    const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+    const double2 Qe = FWD ? dpp2<kWaveShr1>(Qi, one) : dpp2<kWaveShl1>(Qi, one);
+    const double2 ze = FWD ? dpp2<kWaveShr1>(zi, zero) : dpp2<kWaveShl1>(zi, zero);
+    if (lane == (FWD ? 63 : 0)) {
+        slot[wave][0] = Qi;
+        slot[wave][1] = zi;
+    }
+    __syncthreads();
+    // carry into this wave: compose the totals of the waves before it
+    double2 Z = make_double2(0.0, 0.0);
+    if (FWD) {
+        for (int v = 0; v < wave; v++) {
+            const double2 q = slot[v][0], z = slot[v][1];
+            Z = make_double2(z.x + q.x * Z.x - q.y * Z.y, z.y + q.x * Z.y + q.y * Z.x);
+        }
+    } else {
+        for (int v = nw - 1; v > wave; v--) {
+            const double2 q = slot[v][0], z = slot[v][1];
+            Z = make_double2(z.x + q.x * Z.x - q.y * Z.y, z.y + q.x * Z.y + q.y * Z.x);
+        }
+    }
+    return make_double2(ze.x + Qe.x * Z.x - Qe.y * Z.y, ze.y + Qe.x * Z.y + Qe.y * Z.x);
+}
+
+} // namespace tdse
