@@ -70,15 +70,17 @@ def _stream(t):
     return c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
-def run_states(geometry, F, n_steps, k0, n_run, init, save, save_every, out=None):
-    """Partial forward runs: rows of F (n_inst, L) are gated fields; init (n_inst, state) or
-    None; save (n_inst, m, state) zero-initialised, complex128 viewed as float64 pairs."""
+def run_states(geometry, F, n_steps, k0, n_run, init, save, save_every, out=None, init_slot=0):
+    """Partial forward runs: rows of F (n_inst, L) are gated fields; init (n_inst, m, state, 2)
+    or None, the start state of instance i being init[i, init_slot]; save (n_inst, m', state, 2)
+    zero-initialised; states are complex128 viewed as float64 pairs in the tiled layout."""
     L = _bind()
     n_inst, Lf = F.shape
     S = state_length(geometry)
     geoms = [geometry] * n_inst
     init_p = c_void_p(init.data_ptr()) if init is not None else c_void_p()
-    init_off = _ll([i * 2 * S for i in range(n_inst)])
+    per_init = init.shape[1] * 2 * S if init is not None else 0
+    init_off = _ll([i * per_init + init_slot * 2 * S for i in range(n_inst)])
     per_save = save.shape[1] * 2 * S if save is not None else 0
     save_p = c_void_p(save.data_ptr()) if save is not None else c_void_p()
     out_p = c_void_p(out.data_ptr()) if out is not None else c_void_p()
@@ -143,8 +145,7 @@ class TDSEDipole(torch.autograd.Function):
             k_hi = min((m + 1) * K, N - 1)
             if k_hi <= k_lo:
                 continue
-            init = None if m == 0 else ckpt[:, m - 1]
-            run_states(geometry, F, N, k_lo, k_hi - k_lo, init.contiguous() if init is not None else None, block, 1)
+            run_states(geometry, F, N, k_lo, k_hi - k_lo, None if m == 0 else ckpt, block, 1, init_slot=m - 1)
             run_adjoint(geometry, F, gobs, block, kappa, grad, k_hi, k_lo)
         dE = torch.zeros_like(F)
         dE[:, :N] = grad * mask[:, :N]

@@ -37,6 +37,7 @@ using namespace tdse;
 // psi is updated in place: the segments of a step are solved in order, and the only
 // chunks a segment reads after the previous segment has overwritten them are its H
 // left-halo chunks, whose old values the previous segment saved to LDS (halo_old).
+template <bool SAVE, bool SNAP>
 __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
                               double2 *__restrict__ psi, double Fdt, double2 rot, Acc &acc, double *snap_row,
                               double2 (*slot_f)[2], double2 (*slot_b)[2], double2 *halo_in, double2 *halo_out,
@@ -163,7 +164,9 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
         const double2 v = cmul(e, P[i]);
         P[i] = v;
         psi[base + 64 * i] = v;
-        if (save_to) save_to[base + 64 * i] = v;
+        if constexpr (SAVE) {
+            if (save_to) save_to[base + 64 * i] = v;
+        }
         const double n2 = fma(v.x, v.x, v.y * v.y);
         const double xj = __dsub_rn(__dmul_rn((double)(j0 + i), G.dx), G.xmax);
         acc.pos = fma(n2, xj, acc.pos);
@@ -188,7 +191,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
             if (j >= G.pint_lo && j <= G.pint_hi) acc.pint += fma(P[i].x, P[i].x, P[i].y * P[i].y);
         }
     }
-    if (snap_row) {
+    if (SNAP && snap_row) {
 #pragma unroll
         for (int i = 0; i < C; i++) {
             if (j0 + i < G.nx) {
@@ -201,6 +204,9 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
 
 } // namespace
 
+// SAVE: some instance stores states (partial runs); SNAP: some instance takes snapshots.
+// Both are off for plain batches, which then carry neither store in the step loop.
+template <bool SAVE, bool SNAP>
 __global__ void __launch_bounds__(64 * TDSE_MAX_WAVES)
 tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst, int *counter,
                       double2 *work, long long work_stride)
@@ -252,7 +258,7 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
                 F = fk;
             }
             double2 *save_to = nullptr;
-            if (I.save && (k - k_start + 1) % I.save_every == 0)
+            if (SAVE && I.save && (k - k_start + 1) % I.save_every == 0)
                 save_to = I.save + (long long)((k - k_start + 1) / I.save_every - 1) * npad;
             const double Fdt = F * G.dt;
             double rs, rc;
@@ -260,13 +266,13 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
             const double2 rot = make_double2(rc, rs);
             // snapshots, prop.c:248-255: row i_snap gets psi after step k when k % every == 0
             double *snap_row = nullptr;
-            if (I.snap && k % I.snap_every == 0 && i_snap < I.n_snap - 1) {
+            if (SNAP && I.snap && k % I.snap_every == 0 && i_snap < I.n_snap - 1) {
                 snap_row = I.snap + (long long)i_snap * 2 * G.nx;
                 i_snap++;
             }
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
             for (int s = 0; s < nseg; s++)
-                solve_segment(G, s, own_chunks, wave, lane, nw, buf, Fdt, rot, acc, snap_row, slot_f, slot_b,
+                solve_segment<SAVE, SNAP>(G, s, own_chunks, wave, lane, nw, buf, Fdt, rot, acc, snap_row, slot_f, slot_b,
                               halo[s & 1], halo[(s + 1) & 1], save_to);
             // a final snapshot that coincides with the last step is written after the loop
             const double pr = wave_sum(acc.pr), pi = wave_sum(acc.pi), pint = wave_sum(acc.pint);
@@ -316,3 +322,12 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
         __syncthreads();
     }
 }
+
+template __global__ void tdse_propagate_kernel<false, false>(const TdseGeom *, const TdseInst *, const int *, int, int *,
+                                                             double2 *, long long);
+template __global__ void tdse_propagate_kernel<true, false>(const TdseGeom *, const TdseInst *, const int *, int, int *,
+                                                            double2 *, long long);
+template __global__ void tdse_propagate_kernel<false, true>(const TdseGeom *, const TdseInst *, const int *, int, int *,
+                                                            double2 *, long long);
+template __global__ void tdse_propagate_kernel<true, true>(const TdseGeom *, const TdseInst *, const int *, int, int *,
+                                                           double2 *, long long);

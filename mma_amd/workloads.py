@@ -102,3 +102,24 @@ def uneven_batch(n_inst=8192, seed=2024, dt=0.25, nr_range=(2000, 16000), cycles
         out.append(dict(num_r=int(2 * half[i]), n_cycles=int(cyc[i]), intensity=float(inten[i]),
                         wavelength=float(lam[i]), t=t, field=sin2_pulse(t, field_au(inten[i]), w, int(cyc[i]))))
     return out
+
+
+# grid of the gradient workload: x in [-400, 400] a.u., dx = 0.1 (Nx = 8001), dt = 0.1 a.u.
+ADJOINT_GRID = dict(num_r=8000, dx=0.1, dt=0.1, a=SOFT_COULOMB_A["Ar"], e_guess=SOFT_COULOMB_E["Ar"], cv=1e-25,
+                    x_int=2.0)
+
+
+def adjoint_batch(n_inst=1024, n_t=40000, dt=0.1, w0=100e-6, wavelength=800e-9, intensity=1.5e14,
+                  n_cycles=30, r_max_w0=1.5):
+    """Gradient workload: n_inst fields of n_t samples (dt apart) across the focal plane of a
+    Gaussian beam (r in linspace(0, r_max_w0 w0, n_inst), amplitude exp(-r^2/w0^2)), each a
+    sin^2 pulse of n_cycles at the given wavelength.  Returns t (n_t,), fields (n_inst, n_t)."""
+    w = omega_au(wavelength)
+    T = n_cycles * 2.0 * math.pi / w
+    if T > (n_t - 1) * dt:
+        raise ValueError("pulse longer than the sampled window")
+    t = np.arange(n_t) * dt
+    r = np.linspace(0.0, r_max_w0 * w0, n_inst)
+    amp = field_au(intensity) * np.exp(-(r / w0) ** 2)
+    env = np.where(t <= T, np.sin(math.pi * t / T) ** 2, 0.0)
+    return t, amp[:, None] * (env * np.sin(w * t))[None, :]

@@ -177,3 +177,90 @@ def test_gpu_gradient_independent_of_checkpoint_interval(gpu_case):
         print("K", K, flush=True)
     for g in grads[1:]:
         assert np.array_equal(g, grads[0])
+
+
+# ------------------------------------------------- gradient workload (Nx = 8001, dt = 0.1)
+
+def _workload_geometry():
+    from mma_amd import tdse, workloads as W
+    g = W.ADJOINT_GRID
+    return tdse.Geometry(g["num_r"], g["dx"], g["dt"], g["a"], g["e_guess"], g["cv"], g["x_int"])
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_workload_vs_oracle():
+    """Two members of the gradient workload on its grid, on a 4000-sample window, against
+    the oracle gradient (harmonic-yield loss over harmonics 21-31)."""
+    from mma_amd import adjoint as AD, tdse, workloads as W
+    from oracle import adjoint as OA
+    g = W.ADJOINT_GRID
+    geo = _workload_geometry()
+    x, psi0 = geo.ground_state()
+    t, f = W.adjoint_batch(n_inst=8, n_t=4000, n_cycles=3)
+    f = tdse.pad_fields(f[[0, 5]])
+    n = f.shape[1] - 1
+    print("workload vs oracle: Nx %d dt %g steps %d" % (g["num_r"] + 1, g["dt"], n), flush=True)
+    win = AD.harmonic_window(n, g["dt"], n * g["dt"], W.omega_au(800e-9), 21, 31)
+    E = torch.tensor(f, device="cuda", requires_grad=True)
+    s = AD.TDSEDipole.apply(E, geo, n)
+    AD.harmonic_yield(s, g["dt"], win.cuda()).sum().backward()
+    got = E.grad.cpu().numpy()
+    for i in range(2):
+        F, kg = OA.gate(f[i])
+        ref_s = OA.dipole(g["num_r"], g["dx"], g["a"], g["x_int"], g["dt"], psi0, x, F, n)
+        gs = OA.harmonic_yield_grad(ref_s, g["dt"], win.numpy().astype(float))
+        ref_g = OA.gradient(g["num_r"], g["dx"], g["a"], g["x_int"], g["dt"], psi0, x, F, n, gs)
+        ref_g[:kg + 1] = 0.0
+        es = rel_err(s[i].detach().cpu().numpy(), ref_s)
+        eg = rel_err(got[i, :n], ref_g)
+        print(i, "dipole %.2e grad %.2e" % (es, eg), flush=True)
+        assert es < 1e-8 and eg < 1e-8
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_workload_finite_differences_64_directions():
+    """Every 16th member of the 1024-instance gradient workload (40000 field samples, Nx = 8001,
+    dt = 0.1), one seeded random unit direction each (zero up to the gating point): central
+    differences (h = 1e-6) of the harmonic yield over harmonics 21-31 against the adjoint
+    directional derivative.  The error is measured against |grad| (|direction| = 1), so a
+    direction nearly orthogonal to the gradient is not judged by its own tiny derivative.
+    Members 0 and 20 are also compared with the oracle gradient over the full length."""
+    from mma_amd import adjoint as AD, tdse, workloads as W
+    from oracle import adjoint as OA
+    g = W.ADJOINT_GRID
+    geo = _workload_geometry()
+    t, f = W.adjoint_batch(n_inst=1024)
+    f = tdse.pad_fields(f[::16])
+    n = f.shape[1] - 1
+    win = AD.harmonic_window(n, g["dt"], n * g["dt"], W.omega_au(800e-9), 21, 31).cuda()
+    print("64 directions: Nx %d steps %d" % (g["num_r"] + 1, n), flush=True)
+
+    def losses(E):
+        return AD.harmonic_yield(AD.TDSEDipole.apply(E, geo, n), g["dt"], win)
+
+    E0 = torch.tensor(f, device="cuda")
+    E = E0.clone().requires_grad_(True)
+    losses(E).sum().backward()
+    grad = E.grad.clone()
+    print("gradient done", flush=True)
+    _, mask = AD.gate(E0)
+    V = torch.tensor(np.random.default_rng(11).standard_normal(f.shape), device="cuda") * mask
+    V = V / V.norm(dim=1, keepdim=True)
+    h = 1e-6
+    with torch.no_grad():
+        fd = ((losses(E0 + h * V) - losses(E0 - h * V)) / (2 * h)).cpu().numpy()
+    ad = (grad * V).sum(dim=1).cpu().numpy()
+    err = np.abs(fd - ad) / grad.norm(dim=1).cpu().numpy()
+    print("max |fd - ad| / |grad| = %.3e, median %.3e" % (err.max(), np.median(err)), flush=True)
+    assert np.all(err <= 1e-6)
+    x, psi0 = geo.ground_state()
+    got = grad.cpu().numpy()
+    for i in (0, 20):
+        F, kg = OA.gate(f[i])
+        ref_s = OA.dipole(g["num_r"], g["dx"], g["a"], g["x_int"], g["dt"], psi0, x, F, n)
+        gs = OA.harmonic_yield_grad(ref_s, g["dt"], win.cpu().numpy().astype(float))
+        ref_g = OA.gradient(g["num_r"], g["dx"], g["a"], g["x_int"], g["dt"], psi0, x, F, n, gs)
+        ref_g[:kg + 1] = 0.0
+        e = rel_err(got[i, :n], ref_g)
+        print("member", i, "full-length gradient vs oracle %.2e" % e, flush=True)
+        assert e < 1e-8
