@@ -1,4 +1,4 @@
-"""Minimal read-only HDF5 reader (h5py is not installed in this image).
+"""Minimal read-only HDF5 reader, used where h5py is not available.
 
 Covers what the reference's data files use: superblock v0/v1, version-1 object
 headers (with continuation blocks), old-style groups (symbol-table B-tree v1 +

@@ -21,7 +21,7 @@ sys.path.insert(0, HERE)
 
 from oracle import ctdse as O  # noqa: E402
 from mma_amd import workloads as W  # noqa: E402
-import h5lite  # noqa: E402
+from mma_amd import h5lite  # noqa: E402
 import layout_probe  # noqa: E402
 
 REF_H5 = "/root/reference/1DTDSE/ionization.h5"
