@@ -1,4 +1,5 @@
-"""Builds the native library mma_amd/lib/libsingleTDSE.so for gfx950 with hipcc.
+"""Builds the native libraries for gfx950 with hipcc: mma_amd/lib/libsingleTDSE.so (TDSE)
+and mma_amd/lib/libmmahankel.so (Hankel far-field transform).
 
 Usage: python -m mma_amd.build   (or mma_amd.build.build() from Python)
 """
@@ -27,9 +28,17 @@ SOURCES = [
     ("tdse_propagate.hip", []),
     ("tdse_adjoint.hip", []),
     ("ctdse_abi.cpp", ["-x", "hip"]),
+    ("hankel_medium.hip", []),
+    ("hankel_contract.hip", []),
+    ("hankel_gemm.hip", []),
+    ("hankel_abi.cpp", ["-x", "hip"]),
 ]
 
-LIBS = {"libsingleTDSE.so": [s for s, _ in SOURCES]}
+LIBS = {
+    "libsingleTDSE.so": ["tdse_setup.hip", "tdse_analysis.hip", "tdse_propagate.hip", "tdse_adjoint.hip",
+                         "ctdse_abi.cpp"],
+    "libmmahankel.so": ["hankel_medium.hip", "hankel_contract.hip", "hankel_gemm.hip", "hankel_abi.cpp"],
+}
 
 
 def _run(cmd, verbose):

@@ -1,0 +1,265 @@
+// C ABI of libmmahankel.so (include/hankel.h): argument checks, launch shapes and the error
+// status; every entry point returns after the work it queued on `stream` has finished.
+#include <algorithm>
+#include <cstdint>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "hankel_internal.h"
+
+namespace {
+
+thread_local int g_status = HK_OK;
+thread_local std::string g_message;
+
+int fail(int code, const std::string &msg)
+{
+    g_status = code;
+    g_message = msg;
+    return code;
+}
+void clear_status()
+{
+    g_status = HK_OK;
+    g_message.clear();
+}
+
+#define HK_HIP(call)                                                                                         \
+    do {                                                                                                     \
+        const hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return fail(HK_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// shared_python/units.py:3-20 and XUV_refractive_index.py:123, in the reference's operation order
+HkConst make_constants()
+{
+    const double hbar = 1.0545718e-34, inverse_alpha_fine = 137.035999139, c_light = 299792458.0;
+    const double elcharge = 1.602176565e-19, elmass = 9.10938356e-31;
+    const double r_bohr = hbar * inverse_alpha_fine / (c_light * elmass);
+    const double alpha_fine = 1.0 / inverse_alpha_fine;
+    const double alpha2 = alpha_fine * alpha_fine;
+    const double r_electron = r_bohr * alpha2;
+    HkConst K;
+    K.c_light = c_light;
+    K.timeau = (elmass * (r_bohr * r_bohr)) / hbar;
+    K.ev_per_au = elcharge / ((elmass * alpha2) * (c_light * c_light));
+    K.lam_num = 2.0 * M_PI * hbar;
+    K.lam_den = (elmass * c_light) * alpha2;
+    K.n_ref_re = 2.7e25 * r_electron;
+    return K;
+}
+
+const double kInvC = 1.0 / 299792458.0;
+
+int check_planes(const hk_planes *p)
+{
+    if (!p) return fail(HK_ERR_ARGUMENT, "null plane description");
+    if (p->nz < 1 || p->no < 1 || p->nr < 2 || p->nrho < 1)
+        return fail(HK_ERR_ARGUMENT, "need nz >= 1, no >= 1, nr >= 2, nrho >= 1");
+    if (!p->src || !p->zgrid || !p->ogrid || !p->rgrid || !p->tw || !p->rho)
+        return fail(HK_ERR_ARGUMENT, "null device array");
+    if (p->pre_mode < 0 || p->pre_mode > 2 || (p->pre_mode != 0 && !p->pre))
+        return fail(HK_ERR_ARGUMENT, "pre_mode must be 0 (none), 1 (plane x omega) or 2 (plane x omega x r)");
+    if (p->plane_stride < (long long)p->no * p->nr) return fail(HK_ERR_ARGUMENT, "plane_stride < no * nr");
+    return HK_OK;
+}
+
+// host copies of the small grids of a plane description
+struct HostGrids {
+    std::vector<double> r, z, o;
+};
+
+int host_grids(const hk_planes *p, HostGrids &g)
+{
+    g.r.resize(p->nr);
+    g.z.resize(p->nz);
+    g.o.resize(p->no);
+    HK_HIP(hipMemcpy(g.r.data(), p->rgrid, sizeof(double) * p->nr, hipMemcpyDeviceToHost));
+    HK_HIP(hipMemcpy(g.z.data(), p->zgrid, sizeof(double) * p->nz, hipMemcpyDeviceToHost));
+    HK_HIP(hipMemcpy(g.o.data(), p->ogrid, sizeof(double) * p->no, hipMemcpyDeviceToHost));
+    return HK_OK;
+}
+
+// largest q = (omega / c) r / (distance - z) of the sources; requires r >= 0, omega >= 0 and
+// every plane in front of the screen
+int source_q_max(const hk_planes *p, const HostGrids &g, double *q_max)
+{
+    double rmax = 0.0, omax = 0.0, dmin = INFINITY;
+    for (double v : g.r) {
+        if (!(v >= 0.0)) return fail(HK_ERR_ARGUMENT, "the gridded contraction needs radii >= 0");
+        rmax = std::max(rmax, v);
+    }
+    for (double v : g.o) {
+        if (!(v >= 0.0)) return fail(HK_ERR_ARGUMENT, "the gridded contraction needs frequencies >= 0");
+        omax = std::max(omax, v);
+    }
+    for (double v : g.z) dmin = std::min(dmin, p->distance - v);
+    if (!(dmin > 0.0)) return fail(HK_ERR_ARGUMENT, "every source plane must lie before the screen");
+    *q_max = omax * kInvC / dmin * rmax;
+    return HK_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int hk_last_error(void) { return g_status; }
+const char *hk_last_error_message(void) { return g_message.c_str(); }
+
+int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const double *ogrid, int nr,
+                 const double *rgrid, double *pre, double *renorm, void *stream)
+{
+    clear_status();
+    if (!m || nz < 1 || no < 1 || !zgrid || !ogrid || !pre) return fail(HK_ERR_ARGUMENT, "bad pre-factor arguments");
+    if (m->mode < HK_PRESSURE_SCALAR || m->mode > HK_PRESSURE_ZR) return fail(HK_ERR_ARGUMENT, "unknown pressure mode");
+    if (m->mode != HK_PRESSURE_ZR && !renorm)
+        return fail(HK_ERR_ARGUMENT, "renorm is required for scalar and z-tabulated pressure");
+    if (m->mode != HK_PRESSURE_SCALAR && (m->np_z < 1 || !m->p_z || !m->p_value))
+        return fail(HK_ERR_ARGUMENT, "empty pressure table");
+    if (m->mode == HK_PRESSURE_ZR && (nr < 1 || !rgrid || m->np_r < 1 || !m->p_r))
+        return fail(HK_ERR_ARGUMENT, "z-r pressure table needs the r grids");
+    if ((m->include_dispersion && (m->n_f1 < 1 || !m->e_f1 || !m->f1)) ||
+        (m->include_absorption && (m->n_f2 < 1 || !m->e_f2 || !m->f2)))
+        return fail(HK_ERR_ARGUMENT, "missing scattering-factor table");
+    hipStream_t s = (hipStream_t)stream;
+    int *flag = nullptr;
+    HK_HIP(hipMalloc(&flag, sizeof(int)));
+    int zero = 0, got = 0;
+    hipError_t e = hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, s);
+    const int n = m->mode == HK_PRESSURE_ZR ? no * nr : no;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(hk_prefactor_kernel, dim3((n + 127) / 128), dim3(128), 0, s, make_constants(), *m, nz, zgrid,
+                           no, ogrid, nr, rgrid, reinterpret_cast<double2 *>(pre), renorm, flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&got, flag, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(flag);
+    if (e != hipSuccess) return fail(HK_ERR_DEVICE, std::string("pre-factor kernel: ") + hipGetErrorString(e));
+    if (got == HK_FLAG_RANGE) return fail(HK_ERR_TABLE, "photon energy outside the tabulated scattering factors");
+    if (got == HK_FLAG_F1) return fail(HK_ERR_TABLE, "f1-scattering coefficient not listed for this frequency.");
+    return HK_OK;
+}
+
+int hk_direct(const hk_planes *p, int n_out, const double *weights, int per_plane, double *out, void *stream)
+{
+    clear_status();
+    if (check_planes(p) != HK_OK) return g_status;
+    if (!out || (!per_plane && (n_out < 1 || n_out > 4 || !weights)))
+        return fail(HK_ERR_ARGUMENT, "need out and 1..4 weighted outputs (or per_plane)");
+    // a uniform radial grid lets the kernel rotate cos/sin of the Bessel argument
+    std::vector<double> r(p->nr);
+    HK_HIP(hipMemcpy(r.data(), p->rgrid, sizeof(double) * p->nr, hipMemcpyDeviceToHost));
+    // (every r_j within a few ulps of r_0 + j dr, so rotated angles stay within ~1e-13 rad)
+    const double dr = (r.back() - r.front()) / (p->nr - 1);
+    double rmax = 0.0;
+    for (double v : r) rmax = std::max(rmax, std::fabs(v));
+    bool uniform = dr > 0;
+    for (int j = 1; j < p->nr && uniform; j++)
+        uniform = std::fabs(r[j] - (r.front() + j * dr)) <= 8.0 * 2.220446049250313e-16 * rmax;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid(p->no, (p->nrho + 255) / 256);
+    if (p->src_c64)
+        hipLaunchKernelGGL(hk_direct_kernel<float2>, grid, dim3(256), 0, s, *p, n_out, weights, per_plane,
+                           reinterpret_cast<double2 *>(out), kInvC, (int)uniform, r.front(), dr);
+    else
+        hipLaunchKernelGGL(hk_direct_kernel<double2>, grid, dim3(256), 0, s, *p, n_out, weights, per_plane,
+                           reinterpret_cast<double2 *>(out), kInvC, (int)uniform, r.front(), dr);
+    HK_HIP(hipGetLastError());
+    HK_HIP(hipStreamSynchronize(s));
+    return HK_OK;
+}
+
+int hk_grid_size(double q_max, double rho_max, double tol, int *nq, double *h)
+{
+    clear_status();
+    if (!(q_max > 0) || !(rho_max > 0) || !(tol > 0) || !nq || !h) return fail(HK_ERR_ARGUMENT, "bad grid request");
+    // Lagrange interpolation of P points: |error| <= (rho h)^P max_f |prod (f - d_i)| / P!
+    const int P = HK_GRID_P;
+    double nodal = 1.0, fact = 1.0;
+    for (int i = 0; i < P; i++) {
+        nodal *= std::fabs(0.5 - (double)(i - (P / 2 - 1)));
+        fact *= (double)(i + 1);
+    }
+    const double step = std::pow(tol / (nodal / fact), 1.0 / P);  // allowed rho_max * h
+    const double need = rho_max * q_max / step + P + 1;
+    const long long n = ((long long)std::ceil(need) + 63) / 64 * 64;
+    if (n > 10240) return fail(HK_ERR_UNSUPPORTED, "q grid of " + std::to_string(n) + " points exceeds LDS (10240)");
+    *nq = (int)n;
+    *h = q_max * (1.0 + 1e-12) / (double)(n - P - 1);
+    return HK_OK;
+}
+
+int hk_grid_spread(const hk_planes *p, const double *weights, int nq, double h, int type, void *out, long long ld,
+                   long long row0, void *stream)
+{
+    clear_status();
+    if (check_planes(p) != HK_OK) return g_status;
+    if (!weights || !out || nq < HK_GRID_P + 2 || nq > 10240 || ld < nq || row0 < 0 || !(h > 0))
+        return fail(HK_ERR_ARGUMENT, "bad spread arguments");
+    if (p->pre_mode == 2) return fail(HK_ERR_UNSUPPORTED, "the gridded contraction needs an r-independent pre-factor");
+    HostGrids g;
+    double q_max = 0.0;
+    if (host_grids(p, g) != HK_OK || source_q_max(p, g, &q_max) != HK_OK) return g_status;
+    if (q_max / h > (double)(nq - HK_GRID_P - 1) * (1.0 + 1e-9))
+        return fail(HK_ERR_ARGUMENT, "q grid too short for these sources (use hk_grid_size)");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t lds = (size_t)nq * sizeof(double2);
+    if (type == HK_F64) {
+        HK_HIP(hipFuncSetAttribute((const void *)hk_spread_kernel<HK_F64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL(hk_spread_kernel<HK_F64>, dim3(p->no), dim3(256), lds, s, *p, weights, nq, h, kInvC, out,
+                           ld, row0);
+    } else if (type == HK_F32) {
+        HK_HIP(hipFuncSetAttribute((const void *)hk_spread_kernel<HK_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL(hk_spread_kernel<HK_F32>, dim3(p->no), dim3(256), lds, s, *p, weights, nq, h, kInvC, out,
+                           ld, row0);
+    } else {
+        return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");
+    }
+    HK_HIP(hipGetLastError());
+    HK_HIP(hipStreamSynchronize(s));
+    return HK_OK;
+}
+
+int hk_grid_bessel(int nrho, const double *rho, int nq, double h, int type, void *out, long long ld, void *stream)
+{
+    clear_status();
+    if (nrho < 1 || !rho || nq < 1 || !out || ld < nq || !(h > 0)) return fail(HK_ERR_ARGUMENT, "bad Bessel arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const long long n = (long long)nrho * nq;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (type == HK_F64)
+        hipLaunchKernelGGL(hk_bessel_kernel<HK_F64>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
+    else if (type == HK_F32)
+        hipLaunchKernelGGL(hk_bessel_kernel<HK_F32>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
+    else
+        return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");
+    HK_HIP(hipGetLastError());
+    HK_HIP(hipStreamSynchronize(s));
+    return HK_OK;
+}
+
+int hk_gemm(int kind, int m, int n, int k, const float *a, long long lda, const float *b, long long ldb, double *out,
+            long long ldo, void *stream)
+{
+    clear_status();
+    if (m < 1 || n < 1 || k < 64 || k % 64 != 0 || !a || !b || !out || lda < k || ldb < k || ldo < n || lda % 4 != 0 ||
+        ldb % 4 != 0 || ((uintptr_t)a & 15) != 0 || ((uintptr_t)b & 15) != 0)
+        return fail(HK_ERR_ARGUMENT, "GEMM needs k a multiple of 64, 16-byte aligned operands, strides multiple of 4");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)((n + 127) / 128), (unsigned)((m + 127) / 128));
+    if (kind == HK_GEMM_F32)
+        hipLaunchKernelGGL(hk_gemm_kernel<false>, grid, dim3(256), 0, s, m, n, k, a, lda, b, ldb, out, ldo);
+    else if (kind == HK_GEMM_BF16X3)
+        hipLaunchKernelGGL(hk_gemm_kernel<true>, grid, dim3(256), 0, s, m, n, k, a, lda, b, ldb, out, ldo);
+    else
+        return fail(HK_ERR_ARGUMENT, "unknown GEMM kind");
+    HK_HIP(hipGetLastError());
+    HK_HIP(hipStreamSynchronize(s));
+    return HK_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,36 @@
+// Shared declarations of the Hankel kernels and their host launcher.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hankel.h"
+
+// Photon/medium constants of the Python unit set (shared_python/units.py:3-20,
+// XUV_refractive_index.py:123), evaluated on the host in the reference's operation order.
+struct HkConst {
+    double c_light;
+    double timeau;      // omega_au = omega_SI * timeau
+    double ev_per_au;   // E[eV] = omega_au / ev_per_au   (mynumerics.py:313,326)
+    double lam_num;     // lambda_SI = lam_num / (omega_au * lam_den)  (mynumerics.py:323)
+    double lam_den;
+    double n_ref_re;    // N_ref * r_electron_classical
+};
+
+enum HkErrFlag { HK_FLAG_OK = 0, HK_FLAG_RANGE = 1, HK_FLAG_F1 = 2 };
+
+__global__ void hk_prefactor_kernel(HkConst K, hk_medium m, int nz, const double *zgrid, int no, const double *ogrid,
+                                    int nr, const double *rgrid, double2 *pre, double *renorm, int *flag);
+
+template <typename S>
+__global__ void hk_direct_kernel(hk_planes p, int n_out, const double *weights, int per_plane, double2 *out,
+                                 double inv_c, int uniform_r, double r0, double dr);
+
+template <int TYPE>
+__global__ void hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double inv_c, void *out,
+                                 long long ld, long long row0);
+
+template <int TYPE>
+__global__ void hk_bessel_kernel(int nrho, const double *rho, int nq, double h, void *out, long long ld);
+
+template <bool SPLIT>
+__global__ void hk_gemm_kernel(int m, int n, int k, const float *a, long long lda, const float *b, long long ldb,
+                               double *out, long long ldo);

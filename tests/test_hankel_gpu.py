@@ -1,0 +1,196 @@
+"""GPU parity of the Hankel far-field stage (libmmahankel.so through mma_amd.hankel) against
+the FP64 oracle (oracle/hankel.py) and closed forms."""
+import math
+
+import numpy as np
+import pytest
+import torch
+from scipy import special
+
+from conftest import rel_err
+from oracle import hankel as OH
+
+pytestmark = pytest.mark.gpu
+OMEGA0 = 2.0 * math.pi * OH.C_LIGHT / 800e-9
+
+
+def _l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_bessel_j0_fp64():
+    """The device J0 (Chebyshev below 25, Hankel asymptotics above) against scipy over the
+    arguments of the far-field integral (0 .. 700)."""
+    from mma_amd import hankel as H
+    from ctypes import byref, c_double
+    nq, h = 4096, 700.0 / 4087
+    rho = torch.tensor([1.0, 0.37], dtype=torch.float64, device="cuda")
+    out = torch.empty((2, nq), dtype=torch.float64, device="cuda")
+    H._check(H._L().hk_grid_bessel(2, H._ptr(rho), nq, h, H.HK_F64, H._ptr(out), nq, H._stream()))
+    q = (np.arange(nq) - 4) * h
+    for i, r in enumerate((1.0, 0.37)):
+        ref = special.jv(0, np.abs(r * q))
+        err = np.max(np.abs(out[i].cpu().numpy() - ref))
+        print("J0 max abs error %.2e" % err, flush=True)
+        assert err < 3e-15
+
+
+@pytest.mark.parametrize("case", ["scalar", "z", "zr", "vacuum"])
+def test_prefactor_tables(case):
+    from mma_amd import hankel as H
+    z = np.linspace(0.0, 5e-3, 17)
+    r = np.linspace(0.0, 1e-4, 9)
+    o = OMEGA0 * np.linspace(21, 61, 12)
+    kw = dict(absorption_tables="Henke", dispersion_tables="NIST")
+    if case == "scalar":
+        args = ("Ar", 0.6)
+    elif case == "z":
+        zp = np.linspace(-1e-3, 6e-3, 57)
+        args = ("Ar", {"zgrid": zp, "value": 0.3 + 0.7 * np.exp(-((zp - 2e-3) / 1.5e-3) ** 2)})
+    elif case == "zr":
+        zp = np.linspace(0.0, 5e-3, 23)
+        rp = np.linspace(0.0, 1.2e-4, 7)
+        args = ("Ar", {"zgrid": zp, "rgrid": rp, "value": 0.5 + np.outer(np.sin(zp * 600), np.cos(rp * 1e4)) * 0.3})
+    else:
+        args = ("vacuum", 1.0)
+        kw = dict(include_absorption=False, include_dispersion=False)
+    pre_f, ren_f = H.get_propagation_pre_factor_function(z, r, o, *args, effective_IR_refrective_index=1.0002, **kw)
+    pre_o, ren_o = OH.prefactor_tables(z, r, o, *args, n_ir=1.0002, **{k.replace("effective_IR_refrective_index",
+                                                                                   "n_ir"): v for k, v in kw.items()})
+    for kz in (0, 7, 16):
+        got = pre_f(kz)
+        ref = pre_o[kz].T if case == "zr" else np.outer(np.ones(len(r)), pre_o[kz])
+        e = rel_err(got, ref)
+        print(case, kz, "pre %.2e" % e, flush=True)
+        assert e < 1e-12
+        if case != "zr":
+            assert rel_err(ren_f(kz), ren_o[kz]) < 1e-12
+    if case == "zr":
+        assert ren_f is None
+
+
+def test_prefactor_errors():
+    from mma_amd import hankel as H
+    z = np.linspace(0, 1e-3, 3)
+    r = np.linspace(0, 1e-4, 3)
+    with pytest.raises(ValueError):  # 10.8 eV: Henke lists no f1 there
+        H.get_propagation_pre_factor_function(z, r, OMEGA0 * np.array([7.0, 9.0]), "Ar", 1.0)
+    with pytest.raises(KeyError):  # the reference fails to find 'vacuum_Henke' tables too
+        H.get_propagation_pre_factor_function(z, r, OMEGA0 * np.array([21.0]), "vacuum", 1.0)
+    with pytest.raises(NotImplementedError):
+        H.get_propagation_pre_factor_function(z, r, OMEGA0 * np.array([21.0]), "Ar",
+                                              {"rgrid": r, "value": np.ones(3)})
+
+
+@pytest.mark.parametrize("uniform,near", [(True, True), (False, True), (True, False)])
+def test_single_plane_transform(uniform, near):
+    from mma_amd import hankel as H
+    rng = np.random.default_rng(3)
+    nr = 200
+    r = np.linspace(0, 1.5e-4, nr) if uniform else np.sort(np.concatenate([[0.0], rng.uniform(0, 1.5e-4, nr - 1)]))
+    o = OMEGA0 * np.linspace(11, 61, 9)
+    F = (rng.standard_normal((9, nr)) + 1j * rng.standard_normal((9, nr))) * np.exp(-(r / 6e-5) ** 2)
+    rho = np.linspace(0, 7e-3, 301)
+    pre = np.exp(1j * rng.uniform(0, 6, (nr, 9)))
+    got = H.HankelTransform(o, r, F, 0.98, rho, near_field_factor=near, pre_factor=pre)
+    ref = OH.hankel_plane(o, r, F, 0.98, rho, near_field_factor=near, pre=pre)
+    e = rel_err(got, ref)
+    print("single plane uniform=%s near=%s: %.2e" % (uniform, near, e), flush=True)
+    assert got.shape == (9, 301) and e < 1e-12
+
+
+def test_gaussian_pair_closed_form():
+    from mma_amd import hankel as H
+    r = np.linspace(0.0, 2e-4, 8001)
+    o = OMEGA0 * np.array([11.0, 31.0, 61.0])
+    a = 1.0 / (3e-5) ** 2
+    F = np.exp(-a * r ** 2)[None, :] * np.ones((3, 1))
+    rho = np.linspace(0.0, 7e-3, 513)
+    got = H.HankelTransform(o, r, F, 1.0, rho, near_field_factor=True)
+    for i, om in enumerate(o):
+        k = om / OH.C_LIGHT
+        aa = a + 1j * k / 2.0
+        ref = np.exp(-(k * rho) ** 2 / (4 * aa)) / (2 * aa)
+        assert np.max(np.abs(got[i] - ref)) <= 2e-7 * np.max(np.abs(ref))
+
+
+def _long_case(nz=9, no=24, nr=160, c64=False, seed=5):
+    rng = np.random.default_rng(seed)
+    zall = np.linspace(0.0, 5e-3, nz + 1)  # the provider drops the last plane
+    r = np.linspace(0.0, 2e-4, nr)
+    o = OMEGA0 * np.linspace(21, 61, no)
+    w = 4e-5 * (1 + 0.3 * np.linspace(0, 1, nz + 1))[:, None, None]
+    S = (np.exp(-(r[None, None, :] / w) ** 2) * np.exp(1j * rng.uniform(0, 6.28, (nz + 1, no, 1)))
+         * (1.0 + 0.05 * rng.standard_normal((nz + 1, no, nr))))
+    if c64:
+        S = S.astype(np.complex64)
+    zp = np.linspace(-5e-4, 5.5e-3, 41)
+    pressure = {"zgrid": zp, "value": 0.2 + np.exp(-((zp - 2.5e-3) / 2e-3) ** 2)}
+    rho = np.linspace(0, 7e-3, 96)
+    return zall, r, o, S, pressure, rho
+
+
+def _oracle_long(zall, r, o, S, pressure, rho, cumulative=False):
+    zz, rr, oo, planes = OH.provider_planes(zall, r, o, S.astype(np.complex128))
+    return OH.hankel_long(zz, rr, oo, planes, 1.0, rho, "Ar", pressure, "Henke", True, "NIST", True, 1.0001,
+                          cumulative=cumulative)
+
+
+def test_hankel_long_fp64_matches_oracle():
+    from mma_amd import hankel as H
+    zall, r, o, S, pressure, rho = _long_case()
+    tgt = H.FSources_provider(zall, r, o, FSource=S)
+    res = H.Hankel_long(tgt, 1.0, rho, "Ar", pressure, "Henke", True, "NIST", True, 1.0001,
+                        store_cumulative_result=True, store_non_normalised_cumulative_result=True)
+    ref = _oracle_long(zall, r, o, S, pressure, rho, cumulative=True)
+    for name in ("FF_integrated", "entry_plane_transform", "exit_plane_transform", "cumulative_field"):
+        e = rel_err(getattr(res, name), ref[name])
+        print(name, "%.2e" % e, flush=True)
+        assert e < 1e-12, name
+    assert res.FF_integrated.shape == (len(rho), len(o))
+    assert res.cumulative_field.shape == (len(zall) - 2, len(rho), len(o))
+    # the non-cumulative path (weighted plane sums in one launch) gives the same field
+    tgt = H.FSources_provider(zall, r, o, FSource=S)
+    res2 = H.Hankel_long(tgt, 1.0, rho, "Ar", pressure, "Henke", True, "NIST", True, 1.0001)
+    assert rel_err(res2.FF_integrated, ref["FF_integrated"]) < 1e-12
+
+
+@pytest.mark.parametrize("contraction,tol", [("fp64-grid", 1e-9), ("fp32-mfma", 1e-5), ("bf16x3-mfma", 1e-4)])
+def test_hankel_long_gridded(contraction, tol):
+    from mma_amd import hankel as H
+    zall, r, o, S, pressure, rho = _long_case()
+    ref = _oracle_long(zall, r, o, S, pressure, rho)
+    tgt = H.FSources_provider(zall, r, o, FSource=S)
+    res = H.Hankel_long(tgt, 1.0, rho, "Ar", pressure, "Henke", True, "NIST", True, 1.0001, contraction=contraction)
+    for name in ("FF_integrated", "entry_plane_transform", "exit_plane_transform"):
+        e, e2 = rel_err(getattr(res, name), ref[name]), _l2(getattr(res, name), ref[name])
+        print(contraction, name, "max %.2e  l2 %.2e" % (e, e2), flush=True)
+        assert e < tol, name
+
+
+def test_gemm_kernels_match_fp64():
+    """hk_gemm (FP32 and bf16x3 matrix-core paths) against an FP64 product of the same FP32
+    operands, with ragged m and n (tile edges)."""
+    from mma_amd import hankel as H
+    rng = np.random.default_rng(7)
+    m, n, k = 200, 330, 256
+    A = torch.tensor(rng.standard_normal((m, k)), dtype=torch.float32, device="cuda")
+    B = torch.tensor(rng.standard_normal((n, k)), dtype=torch.float32, device="cuda")
+    ref = (A.double() @ B.double().T).cpu().numpy()
+    for kind, tol in ((H.HK_GEMM_F32, 1e-6), (H.HK_GEMM_BF16X3, 1e-4)):
+        out = torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda")
+        H._check(H._L().hk_gemm(kind, m, n, k, H._ptr(A), k, H._ptr(B), k, H._ptr(out), n, H._stream()))
+        e = rel_err(out.cpu().numpy(), ref)
+        print("gemm kind", kind, "%.2e" % e, flush=True)
+        assert e < tol
+
+
+def test_complex64_sources_and_provider_selection():
+    from mma_amd import hankel as H
+    zall, r, o, S, pressure, rho = _long_case(nz=6, no=12, nr=96, c64=True)
+    tgt = H.FSources_provider(zall, r, o, FSource=S, ko_min=2, ko_step=3, kr_step=2, kz_step=2)
+    res = H.Hankel_long(tgt, 1.0, rho, "Ar", pressure, "Henke", True, "NIST", True)
+    zz, rr, oo, planes = OH.provider_planes(zall, r, o, S.astype(np.complex128), ko_min=2, ko_step=3, kr_step=2,
+                                            kz_step=2)
+    ref = OH.hankel_long(zz, rr, oo, planes, 1.0, rho, "Ar", pressure, "Henke", True, "NIST", True, 1.0)
+    assert rel_err(res.FF_integrated, ref["FF_integrated"]) < 1e-12
