@@ -135,6 +135,101 @@ def adjoint_measure(n_inst, n_t):
             "first_pass_s": list(runs[0][:2]), "checkpoint_every": int(math.ceil(math.sqrt(n)))}
 
 
+def hankel_cpu_baseline(zg, rg, og, rho, src, pre, distance, threads, target_s=6.0):
+    """The FP64 oracle (numpy + scipy J0, the restatement of HankelTransform) on a slice of the
+    far-field workload: a few whole planes, an omega subset, all rho and r; one omega per thread
+    task.  Returns elements/s (element = one (plane, omega, rho, r) term)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import hankel as OH
+    nz, no, nr = src.shape
+    nrho = len(rho)
+    pre_h = pre.cpu().numpy()
+
+    def one(args):
+        k, o = args
+        plane = src[k, o:o + 1].cpu().numpy()
+        OH.hankel_plane(og[o:o + 1], rg, plane, distance - zg[k], rho, True, np.outer(np.ones(nr), pre_h[k, o:o + 1]))
+        return 1
+
+    t0 = time.perf_counter()
+    one((0, 0))
+    per = time.perf_counter() - t0
+    n_tasks = max(threads, int(round(target_s * threads / max(per, 1e-6))))
+    tasks = [(k % nz, (k * 37) % no) for k in range(n_tasks)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        list(pool.map(one, tasks))
+    dt = time.perf_counter() - t0
+    elems = n_tasks * nrho * nr
+    return {"value": elems / dt, "unit": "elements/s", "cores": threads, "kind": "port",
+            "sample": "%d (plane, omega) pairs x %d rho x %d r: FP64 oracle (numpy, scipy jv) one pair per task "
+                      "on a %d-thread pool, %.1f s; C3 (%.3g elements) extrapolates to %.0f s"
+                      % (n_tasks, nrho, nr, threads, dt, float(nz) * no * nrho * nr, float(nz) * no * nrho * nr * dt / elems)}
+
+
+def hankel_measure(cpu=True):
+    """Far-field workload: 256 source planes (5 mm Ar cell, z-modulated pressure, NIST f1 /
+    Henke f2) x 2048 frequencies (harmonics 11-61 of 800 nm) x 512 radii (0-200 um) onto 1024
+    screen radii (0-7 mm) at 1 m.  Times the device work of each contraction (pre-factor tables
+    included), and states every result's error against the FP64 direct contraction."""
+    import torch
+    from mma_amd import hankel as H, workloads as W
+    zall, rg, og, rho, src, distance = W.hankel_workload()
+    pressure = W.hankel_pressure()
+    zg = zall[:-1]
+    nz, no, nr = src.shape
+    elems = float(nz) * no * len(rho) * nr
+    dz = zg[1:] - zg[:-1]
+    w = np.zeros(nz)
+    w[:-1] += 0.5 * dz
+    w[1:] += 0.5 * dz
+    Wt = np.stack([w, np.eye(nz)[0], np.eye(nz)[-1]])
+
+    def run(kind):
+        tab = H._PreFactor(zg, rg, og, "Ar", pressure, "Henke", True, "NIST", True, 1.0)
+        if kind == "fp64":
+            return H.direct(src, zg, og, rg, rho, distance, weights=Wt, pre=tab.pre)
+        return H.gridded(src, zg, og, rg, rho, distance, Wt, pre=tab.pre, kind=kind)
+
+    out, res = {}, {}
+    for kind in ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma"):
+        run(kind)  # warm-up (code objects, allocations)
+        torch.cuda.synchronize()
+        reps = 1 if kind == "fp64" else 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = run(kind)
+        torch.cuda.synchronize()
+        out[kind] = (time.perf_counter() - t0) / reps
+        res[kind] = r
+        log("hankel %s: %.2f ms" % (kind, out[kind] * 1e3))
+    ref = res["fp64"]
+    errors = {}
+    for kind in ("fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma"):
+        d = res[kind] - ref
+        errors[kind] = {"max_abs_over_max": float(d.abs().max() / ref.abs().max()),
+                        "frobenius": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))}
+    nq = int(H.grid_size_of(zg, og, rg, rho, distance)[0])
+    gemm_flops = 2.0 * len(rho) * (3 * 2 * no) * nq
+    info = {
+        "workload": "far field: %d planes x %d omega x %d r -> %d rho, complex128 sources, Ar z-modulated cell"
+                    % (nz, no, nr, len(rho)),
+        "elements": elems,
+        "seconds": out,
+        "errors_vs_fp64_direct": errors,
+        "speedup_vs_fp64_direct": {k: out["fp64"] / out[k] for k in out if k != "fp64"},
+        "fp64_direct_j0_per_s": elems / out["fp64"],
+        "fp64_direct_algorithmic_tflops": 4.0 * elems / out["fp64"] / 1e12,
+        "grid_points_q": nq,
+        "gemm_flops": gemm_flops,
+    }
+    if cpu:
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        tab = H._PreFactor(zg, rg, og, "Ar", pressure, "Henke", True, "NIST", True, 1.0)
+        info["cpu_baseline"] = hankel_cpu_baseline(zg, rg, og, rho, src, tab.pre, distance, threads)
+    return info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +243,8 @@ def main():
     ap.add_argument("--adj-inst", type=int, default=1024)
     ap.add_argument("--adj-samples", type=int, default=40000)
     ap.add_argument("--only-adjoint", action="store_true", help="time only the gradient workload")
+    ap.add_argument("--no-hankel", action="store_true")
+    ap.add_argument("--only-hankel", action="store_true", help="time only the far-field workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB three times and exit (HBM counter calibration)")
     args = ap.parse_args()
@@ -159,6 +256,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log("copied %d MiB three times" % args.copy_probe)
+        return
+    if args.only_hankel:
+        print(json.dumps({"hankel": hankel_measure(cpu=not args.no_cpu)}), flush=True)
         return
     if args.only_adjoint:
         import torch
@@ -221,6 +321,12 @@ def main():
         torch.cuda.empty_cache()
         adj = adjoint_measure(args.adj_inst, args.adj_samples)
 
+    hk = None
+    if not args.no_hankel:
+        torch.cuda.empty_cache()
+        hk = hankel_measure(cpu=not args.no_cpu)
+        torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -241,6 +347,7 @@ def main():
         "batched_over_per_instance": (value / single) if single else None,
         "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,
         "adjoint": adj,
+        "hankel": hk,
         "kernel_ms": kern_ms,
         "outputs_finite": finite,
     }

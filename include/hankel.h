@@ -107,10 +107,12 @@ int hk_grid_spread(const hk_planes *p, const double *weights, int nq, double h, 
 int hk_grid_bessel(int nrho, const double *rho, int nq, double h, int type, void *out, long long ld, void *stream);
 
 enum hk_gemm_kind {
-    HK_GEMM_F32 = 1,   /* FP32 MFMA (v_mfma_f32_32x32x2_f32), FP32 accumulate */
-    HK_GEMM_BF16X3 = 2 /* both FP32 operands split into bf16 hi + lo on the fly; three bf16 MFMA
-                          products hi*hi + hi*lo + lo*hi (v_mfma_f32_32x32x16_bf16), FP32 accumulate */
+    HK_GEMM_F32 = 1,    /* FP32 MFMA (v_mfma_f32_32x32x2_f32) */
+    HK_GEMM_BF16X3 = 2, /* both FP32 operands split into bf16 hi + lo on the fly; three bf16 MFMA
+                           products hi*hi + hi*lo + lo*hi (v_mfma_f32_32x32x16_bf16) */
+    HK_GEMM_BF16X6 = 3  /* three-way bf16 split, the six products down to 2^-16 relative */
 };
+/* All kinds accumulate each 32-deep K stage in FP32 on the matrix cores and the stages in FP64. */
 
 /* out[m][n] = sum_k A[m][k] B[n][k] on the matrix cores, written as double with row stride
  * ldo.  A: m x k floats (row stride lda), B: n x k floats (ldb); k a multiple of 64, strides

@@ -1,13 +1,17 @@
 // Real GEMM out[m][n] = sum_k A[m][k] B[n][k] on the CDNA4 matrix cores for the gridded
 // Hankel contraction (A = J0(rho_m Q_l), B = the spread sources, one row per real/imaginary
 // part of each frequency).  Block tile 128 x 128 with four waves of 64 x 64, each wave a 2 x 2
-// grid of 32 x 32 MFMA tiles; K staged through LDS 32 at a time.
-//   SPLIT = false: v_mfma_f32_32x32x2_f32 on the FP32 operands.
-//   SPLIT = true:  each FP32 operand is split while staging into bf16 hi = rn(x) and
-//                  lo = rn(x - hi); three v_mfma_f32_32x32x16_bf16 products hi*hi + hi*lo +
-//                  lo*hi accumulate in FP32 (the lo*lo term, ~2^-16 relative, is dropped).
+// grid of 32 x 32 MFMA tiles; K staged through LDS 32 at a time.  Each stage's MFMA sums
+// (FP32) are added into FP64 accumulators, so FP32 rounding grows with 32 terms, not with K.
+//   KIND 1: v_mfma_f32_32x32x2_f32 on the FP32 operands.
+//   KIND 2: each FP32 operand split while staging into bf16 x = p0 + p1 (+ 2^-16 |x|);
+//           products p0 q0 + p0 q1 + p1 q0 on v_mfma_f32_32x32x16_bf16.
+//   KIND 3: three-way split x = p0 + p1 + p2 (+ 2^-24 |x|); the six products of order
+//           <= 2^-16: p0 q0, p0 q1, p1 q0, p0 q2, p1 q1, p2 q0.
 // MFMA operand lanes: A row = lane % 32, B column = lane % 32, K index from lane / 32;
 // result element v of a lane: row 8 (v / 4) + 4 (lane / 32) + v % 4, column lane % 32.
+#include <type_traits>
+
 #include "hankel_internal.h"
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -17,27 +21,38 @@ namespace {
 constexpr int BM = 128, BN = 128, BK = 32;
 constexpr int kPadF = 1;  // float row padding: conflict-free column reads
 constexpr int kPadH = 8;  // bf16 row padding: keeps 16-byte alignment of 8-element reads
+
+template <int NP>
+struct SplitTiles {
+    __attribute__((aligned(16))) __bf16 a[NP][BM][BK + kPadH];
+    __attribute__((aligned(16))) __bf16 b[NP][BN][BK + kPadH];
+};
+struct FloatTiles {
+    float a[BM][BK + kPadF];
+    float b[BN][BK + kPadF];
+};
 }
 
-template <bool SPLIT>
+template <int KIND>
 __global__ void __launch_bounds__(256)
 hk_gemm_kernel(int M, int N, int K, const float *A, long long lda, const float *B, long long ldb, double *out,
                long long ldo)
 {
-    __shared__ float As[SPLIT ? 1 : BM][BK + kPadF], Bs[SPLIT ? 1 : BN][BK + kPadF];
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[SPLIT ? BM : 1][BK + kPadH], Al[SPLIT ? BM : 1][BK + kPadH];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[SPLIT ? BN : 1][BK + kPadH], Bl[SPLIT ? BN : 1][BK + kPadH];
+    constexpr int NP = KIND == HK_GEMM_BF16X6 ? 3 : 2;
+    constexpr bool SPLIT = KIND != HK_GEMM_F32;
+    __shared__ typename std::conditional<SPLIT, SplitTiles<NP>, FloatTiles>::type T;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
-    f32x16 acc[2][2];
+    double accd[2][2][16];
 #pragma unroll
     for (int ti = 0; ti < 2; ti++)
 #pragma unroll
         for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-            for (int v = 0; v < 16; v++) acc[ti][tj][v] = 0.0f;
+            for (int v = 0; v < 16; v++) accd[ti][tj][v] = 0.0;
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int r32 = lane & 31, kh = lane >> 5;
 
     for (int k0 = 0; k0 < K; k0 += BK) {
         float4 ra[4], rb[4];
@@ -57,38 +72,54 @@ hk_gemm_kernel(int M, int N, int K, const float *A, long long lda, const float *
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 if constexpr (SPLIT) {
-                    const __bf16 ah = (__bf16)av[e], bh = (__bf16)bv[e];
-                    Ah[row][c + e] = ah;
-                    Al[row][c + e] = (__bf16)(av[e] - (float)ah);
-                    Bh[row][c + e] = bh;
-                    Bl[row][c + e] = (__bf16)(bv[e] - (float)bh);
+                    float xa = av[e], xb = bv[e];
+#pragma unroll
+                    for (int q = 0; q < NP; q++) {
+                        const __bf16 pa = (__bf16)xa, pb = (__bf16)xb;
+                        T.a[q][row][c + e] = pa;
+                        T.b[q][row][c + e] = pb;
+                        xa -= (float)pa;
+                        xb -= (float)pb;
+                    }
                 } else {
-                    As[row][c + e] = av[e];
-                    Bs[row][c + e] = bv[e];
+                    T.a[row][c + e] = av[e];
+                    T.b[row][c + e] = bv[e];
                 }
             }
         }
         __syncthreads();
-        const int r32 = lane & 31, kh = lane >> 5;
+        f32x16 acc[2][2];
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+            for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) acc[ti][tj][v] = 0.0f;
         if constexpr (SPLIT) {
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
-                bf16x8 ah[2], al[2], bh[2], bl[2];
+                bf16x8 ap[NP][2], bp[NP][2];
 #pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    const int ra_ = wm * 64 + t * 32 + r32, rb_ = wn * 64 + t * 32 + r32;
-                    ah[t] = *reinterpret_cast<const bf16x8 *>(&Ah[ra_][kk + 8 * kh]);
-                    al[t] = *reinterpret_cast<const bf16x8 *>(&Al[ra_][kk + 8 * kh]);
-                    bh[t] = *reinterpret_cast<const bf16x8 *>(&Bh[rb_][kk + 8 * kh]);
-                    bl[t] = *reinterpret_cast<const bf16x8 *>(&Bl[rb_][kk + 8 * kh]);
-                }
+                for (int q = 0; q < NP; q++)
+#pragma unroll
+                    for (int t = 0; t < 2; t++) {
+                        ap[q][t] = *reinterpret_cast<const bf16x8 *>(&T.a[q][wm * 64 + t * 32 + r32][kk + 8 * kh]);
+                        bp[q][t] = *reinterpret_cast<const bf16x8 *>(&T.b[q][wn * 64 + t * 32 + r32][kk + 8 * kh]);
+                    }
 #pragma unroll
                 for (int ti = 0; ti < 2; ti++)
 #pragma unroll
                     for (int tj = 0; tj < 2; tj++) {
-                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[ti], bh[tj], acc[ti][tj], 0, 0, 0);
-                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ti], bl[tj], acc[ti][tj], 0, 0, 0);
-                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[ti], bh[tj], acc[ti][tj], 0, 0, 0);
+                        f32x16 c = acc[ti][tj];
+                        if constexpr (NP == 3) {  // smallest terms first
+                            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[2][ti], bp[0][tj], c, 0, 0, 0);
+                            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1][ti], bp[1][tj], c, 0, 0, 0);
+                            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][ti], bp[2][tj], c, 0, 0, 0);
+                        }
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1][ti], bp[0][tj], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][ti], bp[1][tj], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][ti], bp[0][tj], c, 0, 0, 0);
+                        acc[ti][tj] = c;
                     }
             }
         } else {
@@ -97,8 +128,8 @@ hk_gemm_kernel(int M, int N, int K, const float *A, long long lda, const float *
                 float a[2], b[2];
 #pragma unroll
                 for (int t = 0; t < 2; t++) {
-                    a[t] = As[wm * 64 + t * 32 + r32][kk + kh];
-                    b[t] = Bs[wn * 64 + t * 32 + r32][kk + kh];
+                    a[t] = T.a[wm * 64 + t * 32 + r32][kk + kh];
+                    b[t] = T.b[wn * 64 + t * 32 + r32][kk + kh];
                 }
 #pragma unroll
                 for (int ti = 0; ti < 2; ti++)
@@ -107,8 +138,13 @@ hk_gemm_kernel(int M, int N, int K, const float *A, long long lda, const float *
                         acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
             }
         }
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+            for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) accd[ti][tj][v] += (double)acc[ti][tj][v];
     }
-    const int r32 = lane & 31, kh = lane >> 5;
 #pragma unroll
     for (int ti = 0; ti < 2; ti++)
 #pragma unroll
@@ -117,11 +153,13 @@ hk_gemm_kernel(int M, int N, int K, const float *A, long long lda, const float *
             for (int v = 0; v < 16; v++) {
                 const long long row = m0 + wm * 64 + ti * 32 + 8 * (v >> 2) + 4 * kh + (v & 3);
                 const long long col = n0 + wn * 64 + tj * 32 + r32;
-                if (row < M && col < N) out[row * ldo + col] = (double)acc[ti][tj][v];
+                if (row < M && col < N) out[row * ldo + col] = accd[ti][tj][v];
             }
 }
 
-template __global__ void hk_gemm_kernel<false>(int, int, int, const float *, long long, const float *, long long,
-                                               double *, long long);
-template __global__ void hk_gemm_kernel<true>(int, int, int, const float *, long long, const float *, long long,
-                                              double *, long long);
+template __global__ void hk_gemm_kernel<HK_GEMM_F32>(int, int, int, const float *, long long, const float *, long long,
+                                                     double *, long long);
+template __global__ void hk_gemm_kernel<HK_GEMM_BF16X3>(int, int, int, const float *, long long, const float *,
+                                                        long long, double *, long long);
+template __global__ void hk_gemm_kernel<HK_GEMM_BF16X6>(int, int, int, const float *, long long, const float *,
+                                                        long long, double *, long long);

@@ -31,6 +31,6 @@ __global__ void hk_spread_kernel(hk_planes p, const double *weights, int nq, dou
 template <int TYPE>
 __global__ void hk_bessel_kernel(int nrho, const double *rho, int nq, double h, void *out, long long ld);
 
-template <bool SPLIT>
+template <int KIND>
 __global__ void hk_gemm_kernel(int m, int n, int k, const float *a, long long lda, const float *b, long long ldb,
                                double *out, long long ldo);

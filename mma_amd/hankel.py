@@ -12,7 +12,9 @@ the source planes, and the gridded contraction on the matrix cores.
                  grid, FP64 spreading, FP64 GEMM);
   'fp32-mfma'    the gridded form with the GEMM in FP32 on the matrix cores;
   'bf16x3-mfma'  the gridded form with the GEMM as three bf16 products (hi*hi + hi*lo + lo*hi)
-                 on the matrix cores, FP32 accumulation.
+                 on the matrix cores;
+  'bf16x6-mfma'  the same with a three-way bf16 split and six products.
+The matrix-core GEMMs accumulate 32-deep stages in FP32 and the stages in FP64.
 Only the trapezoidal radial integrator is accelerated; any other integrator raises
 NotImplementedError (the reference accepts an arbitrary callable).
 """
@@ -32,9 +34,10 @@ C_LIGHT = 299792458.0  # shared_python/units.py:3
 
 HK_OK, HK_ERR_ARGUMENT, HK_ERR_UNSUPPORTED, HK_ERR_DEVICE, HK_ERR_TABLE = 0, 3, 4, 5, 7
 HK_F64, HK_F32 = 0, 1
-HK_GEMM_F32, HK_GEMM_BF16X3 = 1, 2
+HK_GEMM_F32, HK_GEMM_BF16X3, HK_GEMM_BF16X6 = 1, 2, 3
 GRID_TOL = 1e-10  # bound on the J0 interpolation error of the gridded contraction
-CONTRACTIONS = ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma")
+CONTRACTIONS = ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma")
+_GEMM_KIND = {"fp32-mfma": HK_GEMM_F32, "bf16x3-mfma": HK_GEMM_BF16X3, "bf16x6-mfma": HK_GEMM_BF16X6}
 
 
 class hk_medium(Structure):
@@ -307,6 +310,17 @@ def direct(src, zgrid, ogrid, rgrid, rho, distance, weights=None, near_field=Tru
     return out
 
 
+def grid_size_of(zgrid, ogrid, rgrid, rho, distance, tol=GRID_TOL):
+    """(nq, h) of the q grid the gridded contraction uses for these grids."""
+    og = np.asarray(ogrid, dtype=np.float64)
+    zg = np.asarray(zgrid, dtype=np.float64)
+    q_max = float(np.max(og) / C_LIGHT / np.min(distance - zg) * np.max(np.abs(rgrid)))
+    rho_max = float(np.max(np.abs(rho)))
+    nq, h = c_int(), c_double()
+    _check(_L().hk_grid_size(q_max if q_max > 0 else 1.0, rho_max if rho_max > 0 else 1.0, tol, byref(nq), byref(h)))
+    return nq.value, h.value
+
+
 def gridded(src, zgrid, ogrid, rgrid, rho, distance, weights, near_field=True, pre=None, kind="fp32-mfma",
             tol=GRID_TOL):
     """Gridded contraction: (n_out, Nrho, No) complex sums over planes with (n_out, Nz) weights.
@@ -316,13 +330,7 @@ def gridded(src, zgrid, ogrid, rgrid, rho, distance, weights, near_field=True, p
     keep = []
     p = _planes_struct(src, zgrid, ogrid, rgrid, rho, distance, near_field, pre, 0 if pre is None else 1, keep)
     L = _L()
-    og = np.asarray(ogrid, dtype=np.float64)
-    zg = np.asarray(zgrid, dtype=np.float64)
-    q_max = float(np.max(og) / C_LIGHT / np.min(distance - zg) * np.max(np.abs(rgrid)))
-    rho_max = float(np.max(np.abs(rho)))
-    nq, h = c_int(), c_double()
-    _check(L.hk_grid_size(q_max if q_max > 0 else 1.0, rho_max if rho_max > 0 else 1.0, tol, byref(nq), byref(h)))
-    nq, h = nq.value, h.value
+    nq, h = grid_size_of(zgrid, ogrid, rgrid, rho, distance, tol)
     W = np.atleast_2d(np.asarray(weights, dtype=np.float64))
     n_out = W.shape[0]
     otype = HK_F64 if kind == "fp64-grid" else HK_F32
@@ -338,7 +346,7 @@ def gridded(src, zgrid, ogrid, rgrid, rho, distance, weights, near_field=True, p
         out = Am @ Bm.T  # plain FP64 library GEMM
     else:
         out = torch.empty((p.nrho, Bm.shape[0]), dtype=torch.float64, device="cuda")
-        k = HK_GEMM_F32 if kind == "fp32-mfma" else HK_GEMM_BF16X3
+        k = _GEMM_KIND[kind]
         _check(L.hk_gemm(k, p.nrho, Bm.shape[0], nq, _ptr(Am), nq, _ptr(Bm), nq, _ptr(out), Bm.shape[0], _stream()))
     out = out.view(p.nrho, n_out, p.no, 2).permute(1, 0, 2, 3).contiguous()
     return torch.view_as_complex(out)

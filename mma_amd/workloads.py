@@ -123,3 +123,57 @@ def adjoint_batch(n_inst=1024, n_t=40000, dt=0.1, w0=100e-6, wavelength=800e-9, 
     amp = field_au(intensity) * np.exp(-(r / w0) ** 2)
     env = np.where(t <= T, np.sin(math.pi * t / T) ** 2, 0.0)
     return t, amp[:, None] * (env * np.sin(w * t))[None, :]
+
+
+# far-field workload: harmonics 11-61 of 800 nm, 5 mm of gas, screen at 1 m
+OMEGA0_SI = 2.0 * math.pi * C_LIGHT / 800e-9
+
+
+def hankel_workload(nz=256, no=2048, nr=512, nrho=1024, r_max=200e-6, rho_max=7e-3, length=5e-3, distance=1.0,
+                    w0=100e-6, h_lo=11, h_hi=61, seed=7, complex64=False, device="cuda"):
+    """Near-field XUV sources of a focused Gaussian beam (SI units), generated on `device`.
+
+    The driver (w0, 800 nm, focus at the cell entrance) has intensity
+    I = (w0/w)^2 exp(-2 r^2/w^2) and phase phi = atan(z/zR) - k r^2 z / (2 (zR^2 + z^2)); the
+    harmonic at omega gets amplitude S(omega) (I/I0)^2 and phase (omega/omega0) phi - alpha I/I0
+    plus a seeded random phase per harmonic order, S(omega) a comb of Gaussians at the odd orders
+    h_lo..h_hi.  Returns (zgrid with nz+1 points, rgrid, ogrid, rho, sources (nz, no, nr)
+    complex, distance): the provider keeps zgrid[0:-1], i.e. nz planes."""
+    import torch
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    zgrid = np.linspace(0.0, length, nz + 1)
+    rgrid = np.linspace(0.0, r_max, nr)
+    ogrid = OMEGA0_SI * np.linspace(h_lo, h_hi, no)
+    rho = np.linspace(0.0, rho_max, nrho)
+    orders = np.arange(h_lo, h_hi + 1, 2)
+    ph_q = torch.rand(len(orders), generator=gen, dtype=torch.float64) * 2 * math.pi
+    alpha = 1.0 + torch.rand(len(orders), generator=gen, dtype=torch.float64)
+    k0 = OMEGA0_SI / C_LIGHT
+    z_r = math.pi * w0 ** 2 / 800e-9
+    dt = torch.float64
+    z = torch.tensor(zgrid[:nz], dtype=dt, device=device)[:, None, None]
+    r = torch.tensor(rgrid, dtype=dt, device=device)[None, None, :]
+    om = torch.tensor(ogrid, dtype=dt, device=device)[None, :, None] / OMEGA0_SI  # harmonic order (real)
+    w = w0 * torch.sqrt(1.0 + (z / z_r) ** 2)
+    inten = (w0 / w) ** 2 * torch.exp(-2.0 * r ** 2 / w ** 2)
+    phi = torch.atan(z / z_r) - k0 * r ** 2 * z / (2.0 * (z_r ** 2 + z ** 2))
+    q = torch.tensor(orders, dtype=dt, device=device)
+    sig = 0.15
+    dist = (om[..., None] - q) ** 2  # (1, no, 1, nq)
+    comb_w = torch.exp(-dist / (2 * sig ** 2))
+    env = comb_w.sum(-1)
+    qphase = (comb_w * ph_q.to(device)).sum(-1) / torch.clamp(env, min=1e-300)
+    qalpha = (comb_w * alpha.to(device)).sum(-1) / torch.clamp(env, min=1e-300)
+    phase = om * phi - qalpha * inten + qphase
+    src = (env * inten ** 2) * torch.exp(1j * phase)
+    src = src.to(torch.complex64 if complex64 else torch.complex128).contiguous()
+    return zgrid, rgrid, ogrid, rho, src, distance
+
+
+def hankel_pressure(length=5e-3, n=101, p0=1.0):
+    """z-modulated gas density of a cell with smooth edges: the pressure dict of the reference
+    (keys zgrid, value), p0 [bar] in the middle."""
+    zp = np.linspace(-0.1 * length, 1.1 * length, n)
+    edge = 0.08 * length
+    val = p0 / (1.0 + np.exp(-(zp - 0.05 * length) / edge * 4)) / (1.0 + np.exp((zp - 0.95 * length) / edge * 4))
+    return {"zgrid": zp, "value": val}
