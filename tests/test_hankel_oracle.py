@@ -96,7 +96,10 @@ def test_xuv_index_is_plasma_dispersion_with_f1_electrons():
             tol = 1e-7 * abs(ref) + 4e-16 / H.C_LIGHT
             assert abs(d - ref) <= tol, (gas, harmonic, d, ref)
             if gas != "Xe_NIST":  # NIST lists f2 of Xe from 64 eV
-                assert H.beta_factor_ref(om, gas) > 0
+                # beta and delta = 1 - n_XUV share N r_e lambda^2 / 2 pi: beta / delta = f2 / f1
+                f2 = float(H.getf2(gas, H.omega_si_to_ev(om)))
+                delta = p * H.N_REF * f1 * H.ELCHARGE ** 2 / (2 * eps0 * H.ELMASS * om ** 2)
+                assert abs(p * H.beta_factor_ref(om, gas) / delta - f2 / f1) <= 1e-7 * abs(f2 / f1)
 
 
 def test_scalar_prefactor_closed_form_and_renorm():
