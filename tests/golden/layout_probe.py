@@ -18,11 +18,22 @@ FIELDS = {
 }
 
 
-def probe(include_dir, header):
+HANKEL_FIELDS = {
+    "hk_medium": ["mode", "pressure", "np_z", "np_r", "p_z", "p_r", "p_value", "include_dispersion",
+                  "include_absorption", "n_ir", "n_f1", "n_f2", "e_f1", "f1", "e_f2", "f2"],
+    "hk_planes": ["nz", "no", "nr", "nrho", "src", "src_c64", "plane_stride", "zgrid", "ogrid", "rgrid", "tw", "rho",
+                  "distance", "near_field", "pre", "pre_mode"],
+}
+
+
+def probe(include_dir, header, structs=None):
+    table = dict(FIELDS, **HANKEL_FIELDS)
+    names = list(structs) if structs else list(FIELDS)
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "%s"' % header, "int main(void) {",
              'printf("{");']
     first = True
-    for s, fields in FIELDS.items():
+    for s in names:
+        fields = table[s]
         lines.append('printf("%s\\"%s\\": {\\"size\\": %%zu, \\"fields\\": {", sizeof(%s));' % ("" if first else ",", s, s))
         first = False
         for i, f in enumerate(fields):

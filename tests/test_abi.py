@@ -54,3 +54,25 @@ def test_ctypes_mirror_layout_matches_reference():
         assert ctypes.sizeof(cls) == spec["size"], name
         for f, off in spec["fields"].items():
             assert getattr(cls, f).offset == off, (name, f)
+
+
+def test_hankel_library_exports_every_declared_symbol():
+    from mma_amd import hankel
+    if not os.path.exists(hankel.LIB_PATH):
+        from mma_amd import build
+        build.build()
+    names = declared_functions(os.path.join(ROOT, "include", "hankel.h"))
+    assert {"hk_prefactor", "hk_direct", "hk_grid_spread", "hk_grid_bessel", "hk_gemm"} <= set(names)
+    lib = ctypes.CDLL(hankel.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_hankel_ctypes_structs_match_header():
+    """The ctypes mirrors of hk_medium and hk_planes have the C layout (offsetof probe)."""
+    from mma_amd import hankel as H
+    got = layout_probe.probe(os.path.join(ROOT, "include"), "hankel.h", structs=("hk_medium", "hk_planes"))
+    for name, cls in (("hk_medium", H.hk_medium), ("hk_planes", H.hk_planes)):
+        assert ctypes.sizeof(cls) == got[name]["size"], name
+        for f, off in got[name]["fields"].items():
+            assert getattr(cls, f).offset == off, (name, f)
