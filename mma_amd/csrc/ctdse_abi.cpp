@@ -253,33 +253,21 @@ struct InstSpec {
     int save_every = 1;
 };
 
-// waves per workgroup for a batch: a segment is nw*64 chunks of which 2H are truncation
-// halo; pick the nw that wastes least on the work-weighted geometries, mildly preferring
-// occupancy (TDSE_WAVES_PER_CU waves fit on a CU)
-int pick_waves(const std::vector<const ctdse_geometry *> &geoms, const std::vector<double> &work,
-               const std::vector<const ctdse_geometry *> &inst_geom, int *min_out)
+// Waves per workgroup.  A segment is nw*64 chunks of which 2H are truncation halo, so larger
+// workgroups waste fewer lanes; but every segment starts with loads and every scan ends at a
+// barrier, and those stalls are hidden only by other workgroups on the same CU in a different
+// phase.  Measured on the 4096 x 16001-point batch (lane efficiency in brackets):
+//   8 wave slots per CU (2 waves/SIMD): 4 x 2 WG/CU 3.00 s (0.78), 2 x 4 3.24 s, 3 x 2 3.54 s,
+//                                       8 x 1 3.66 s, 7 x 1 3.68 s (0.89), 6 x 1 4.12 s;
+//   12 wave slots (3 waves/SIMD):       4 x 3 2.51 s (0.78), 3 x 4 3.14 s, 2 x 6 4.32 s,
+//                                       6 x 2 4.39 s (0.87), 5 x 2 5.39 s.
+// Four-wave workgroups win in both settings, so they are used whenever the halo leaves at
+// least one chunk per lane-row of a segment (H <= 96 lanes; the table build caps H lower).
+int pick_waves(int *min_out, int max_halo_lanes)
 {
     int min_nw = 1;
-    for (const ctdse_geometry *g : geoms)
-        while (min_nw * 64 - 2 * g->dev.halo_lanes < 64) min_nw++;
+    while (min_nw * 64 - 2 * max_halo_lanes < 64) min_nw++;
     int nw = std::max(4, min_nw);
-    double best = -1.0;
-    for (int w = std::max(4, min_nw); w <= TDSE_MAX_WAVES; w++) {
-        double used = 0.0, lanes = 0.0;
-        for (size_t i = 0; i < inst_geom.size(); i++) {
-            const TdseGeom &d = inst_geom[i]->dev;
-            const int own = w * 64 - 2 * d.halo_lanes;
-            const int nseg = (d.nchunks + own - 1) / own;
-            used += (double)d.nchunks * work[i];
-            lanes += (double)nseg * w * 64 * work[i];
-        }
-        const double occ = (double)((TDSE_WAVES_PER_CU / w) * w) / TDSE_WAVES_PER_CU;
-        const double score = (used / lanes) * std::pow(occ, 0.3);
-        if (score > best) {
-            best = score;
-            nw = w;
-        }
-    }
     if (const char *env = std::getenv("MMA_TDSE_WAVES")) nw = std::max(min_nw, std::min(TDSE_MAX_WAVES, std::atoi(env)));
     *min_out = min_nw;
     return nw;
@@ -334,14 +322,9 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
     HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    std::vector<double> work(n);
-    std::vector<const ctdse_geometry *> inst_geom(n);
-    for (int i = 0; i < n; i++) {
-        work[i] = steps_of(i);
-        inst_geom[i] = spec[i].geom;
-    }
-    int min_nw = 1;
-    const int nw = pick_waves(geoms, work, inst_geom, &min_nw);
+    int min_nw = 1, max_h = 0;
+    for (const ctdse_geometry *g : geoms) max_h = std::max(max_h, g->dev.halo_lanes);
+    const int nw = pick_waves(&min_nw, max_h);
     int per_cu = std::max(1, TDSE_WAVES_PER_CU / nw);
     if (const char *env = std::getenv("MMA_TDSE_WG_PER_CU")) per_cu = std::max(1, std::atoi(env));
     const int slots = std::max(1, std::min(n, prop.multiProcessorCount * per_cu));
@@ -395,13 +378,14 @@ int adjoint(const std::vector<const ctdse_geometry *> &ig, const std::vector<Tds
     });
     std::vector<TdseGeom> gdev(geoms.size());
     for (size_t k = 0; k < geoms.size(); k++) gdev[k] = geoms[k]->dev;
-    int min_nw = 1;
-    const int nw = pick_waves(geoms, work, ig, &min_nw);
+    int min_nw = 1, max_h = 0;
+    for (const ctdse_geometry *g : geoms) max_h = std::max(max_h, g->dev.halo_lanes);
+    const int nw = pick_waves(&min_nw, max_h);
     int dev = 0;
     HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    const int per_cu = std::max(1, TDSE_WAVES_PER_CU / nw);
+    const int per_cu = std::max(1, TDSE_ADJ_WAVES_PER_CU / nw);
     const int slots = std::max(1, std::min(n, prop.multiProcessorCount * per_cu));
     std::lock_guard<std::mutex> lock(g_launch_mutex);
     DevMem d_geom, d_inst, d_order, d_counter;

@@ -135,7 +135,7 @@ int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const 
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&got, flag, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    hipFree(flag);
+    (void)hipFree(flag);
     if (e != hipSuccess) return fail(HK_ERR_DEVICE, std::string("pre-factor kernel: ") + hipGetErrorString(e));
     if (got == HK_FLAG_RANGE) return fail(HK_ERR_TABLE, "photon energy outside the tabulated scattering factors");
     if (got == HK_FLAG_F1) return fail(HK_ERR_TABLE, "f1-scattering coefficient not listed for this frequency.");

@@ -39,9 +39,9 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
     double Gv[C];
 #pragma unroll
     for (int i = 0; i < C; i++) {
-        P[i] = from_lds ? halo_in[gl * C + i] : kap[base + 64 * i];
-        Gv[i] = G.g[base + 64 * i];
-        IB[i] = G.ib[base + 64 * i];
+        P[i] = from_lds ? halo_in[gl * C + i] : ld2(kap, base + 64 * i);
+        Gv[i] = ld1(G.g, base + 64 * i);
+        IB[i] = ld2(G.ib, base + 64 * i);
     }
     if (save_old) {
 #pragma unroll
@@ -52,12 +52,12 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
     double2 ibL = dpp2<kWaveShr1>(IB[C - 1], zero2);
     if (lane == 0) {
         const long long nb = tdse_chunk_base(min(max(c - 1, -1), G.nchunks)) + 64 * (C - 1);
-        left = gl > 0 ? kap[nb] : zero2;  // gl >= 64 > H: not yet overwritten
-        ibL = gl > 0 ? G.ib[nb] : zero2;
+        left = gl > 0 ? ld2(kap, nb) : zero2;  // gl >= 64 > H: not yet overwritten
+        ibL = gl > 0 ? ld2(G.ib, nb) : zero2;
     }
     if (lane == 63) {
         const long long nb = tdse_chunk_base(min(max(c + 1, -1), G.nchunks));
-        right = gl < nl - 1 ? kap[nb] : zero2;
+        right = gl < nl - 1 ? ld2(kap, nb) : zero2;
     }
     const bool own = gl >= H && gl < nl - H && c < G.nchunks;
 
@@ -81,8 +81,8 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
             // M2' kappa + i (g (k_{j-1} + k_{j+1}) + d k_j)
             double lr = fma(10.0, k0.x, sr) - fma(g, si, d * k0.y);
             double li = fma(10.0, k0.y, si) + fma(g, sr, d * k0.x);
-            const double2 ps = psik[base + 64 * i];
-            const double w = gobs * G.dpot[base + 64 * i];
+            const double2 ps = ld2(psik, base + 64 * i);
+            const double w = gobs * ld1(G.dpot, base + 64 * i);
             lr = fma(-w, ps.x, lr);
             li = fma(-w, ps.y, li);
             const double xj = __dsub_rn(__dmul_rn((double)(j0 + i), G.dx), G.xmax);
@@ -104,7 +104,7 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
             const double hr = -(b.x - g * b.y), hi = b.y + g * b.x;  // conj(h_{j-1})
             y = make_double2(P[i].x + hr * y.x - hi * y.y, P[i].y + hr * y.y + hi * y.x);
         }
-        double2 Q = G.adq[cl + 64];
+        double2 Q = ld2(G.adq, cl + 64);
         wave_scan<true>(Q, y, lane);
         y = block_carry<true>(Q, y, wave, lane, nw, slot_f);
 #pragma unroll
@@ -128,7 +128,7 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
             const double tr = ib.x * P[i].x + ib.y * P[i].y, ti = ib.x * P[i].y - ib.y * P[i].x;  // conj(ib) y
             u = make_double2(tr + pr * u.x - pi * u.y, ti + pr * u.y + pi * u.x);
         }
-        double2 Gt = G.adg[cl + 64];
+        double2 Gt = ld2(G.adg, cl + 64);
         wave_scan<false>(Gt, u, lane);
         u = block_carry<false>(Gt, u, wave, lane, nw, slot_b);
 #pragma unroll
@@ -144,7 +144,7 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
 
     if (!own) return;
 #pragma unroll
-    for (int i = 0; i < C; i++) kap[base + 64 * i] = P[i];
+    for (int i = 0; i < C; i++) st2(kap, base + 64 * i, P[i]);
 }
 
 } // namespace

@@ -18,7 +18,9 @@
 #define TDSE_WAVE 64
 #define TDSE_TILE (TDSE_WAVE * TDSE_CHUNK)
 #define TDSE_MAX_WAVES 8
-#define TDSE_WAVES_PER_CU 8
+#define TDSE_WAVES_PER_SIMD 3      // propagation kernel: register budget 168 VGPRs, 12 waves per CU
+#define TDSE_WAVES_PER_CU (4 * TDSE_WAVES_PER_SIMD)
+#define TDSE_ADJ_WAVES_PER_CU 8    // adjoint kernel: 2 waves per SIMD (224 VGPRs)
 #define TDSE_MAX_HALO 40  // largest truncation depth in chunks (halo saved in LDS per segment)  // occupancy of the propagation kernel (2 waves per SIMD)
 
 // storage offset of chunk c (c >= -64), row i of the chunk

@@ -58,16 +58,16 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     const bool save_old = gl >= own_chunks && gl < own_chunks + H;  // next segment's left halo
 #pragma unroll
     for (int i = 0; i < C; i++) {
-        P[i] = from_lds ? halo_in[gl * C + i] : psi[base + 64 * i];
-        Gv[i] = G.g[base + 64 * i];
-        IB[i] = G.ib[base + 64 * i];
+        P[i] = from_lds ? halo_in[gl * C + i] : ld2(psi, base + 64 * i);
+        Gv[i] = ld1(G.g, base + 64 * i);
+        IB[i] = ld2(G.ib, base + 64 * i);
     }
     if (save_old) {
 #pragma unroll
         for (int i = 0; i < C; i++) halo_out[(gl - own_chunks) * C + i] = P[i];
     }
-    const double2 Qc = G.chq[cl + 64];
-    const double2 Gc = G.chg[cl + 64];
+    const double2 Qc = ld2(G.chq, cl + 64);
+    const double2 Gc = ld2(G.chg, cl + 64);
     // neighbours across the chunk edges: shuffles inside a wave, loads across waves,
     // zero outside the extended window (truncation zone)
     const double2 zero2 = make_double2(0.0, 0.0);
@@ -75,13 +75,13 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     double gL = dpp<kWaveShr1>(Gv[C - 1], 0.0), gR = dpp<kWaveShl1>(Gv[0], 0.0);
     if (lane == 0) {
         const long long nb = tdse_chunk_base(min(max(c - 1, -1), G.nchunks)) + 64 * (C - 1);
-        left = gl > 0 ? psi[nb] : make_double2(0.0, 0.0);  // gl >= 64 > H: not yet overwritten
-        gL = gl > 0 ? G.g[nb] : 0.0;
+        left = gl > 0 ? ld2(psi, nb) : make_double2(0.0, 0.0);  // gl >= 64 > H: not yet overwritten
+        gL = gl > 0 ? ld1(G.g, nb) : 0.0;
     }
     if (lane == 63) {
         const long long nb = tdse_chunk_base(min(max(c + 1, -1), G.nchunks));
-        right = gl < nl - 1 ? psi[nb] : make_double2(0.0, 0.0);
-        gR = gl < nl - 1 ? G.g[nb] : 0.0;
+        right = gl < nl - 1 ? ld2(psi, nb) : make_double2(0.0, 0.0);
+        gR = gl < nl - 1 ? ld1(G.g, nb) : 0.0;
     }
 
     // r' = (M2' - i H') psi
@@ -149,6 +149,9 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
         }
     }
 
+    // keep the loads of the observable tables below this point: hoisted above the passes they
+    // would hold registers through the whole solve
+    asm volatile("" ::: "memory");
     if (gl < H || gl >= nl - H || c >= G.nchunks) return;  // only the segment's own chunks continue
 
     // length-gauge phase exp(+i F dt x_j) (prop.c:199-204) by rotation from the chunk start,
@@ -163,14 +166,14 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     for (int i = 0; i < C; i++) {
         const double2 v = cmul(e, P[i]);
         P[i] = v;
-        psi[base + 64 * i] = v;
+        st2(psi, base + 64 * i, v);
         if constexpr (SAVE) {
-            if (save_to) save_to[base + 64 * i] = v;
+            if (save_to) st2(save_to, base + 64 * i, v);
         }
         const double n2 = fma(v.x, v.x, v.y * v.y);
         const double xj = __dsub_rn(__dmul_rn((double)(j0 + i), G.dx), G.xmax);
         acc.pos = fma(n2, xj, acc.pos);
-        acc.gp = fma(n2, dp[64 * i], acc.gp);
+        acc.gp = fma(n2, ld1(dp, 64 * i), acc.gp);
         e = cmul(e, rot);
     }
     // <psi0|psi> over the rows where psi0 is not negligible
@@ -178,7 +181,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
         const double2 *q0 = G.psi0 + base;
 #pragma unroll
         for (int i = 0; i < C; i++) {
-            const double2 q = q0[64 * i];
+            const double2 q = ld2(q0, 64 * i);
             acc.pr = fma(P[i].x, q.x, fma(P[i].y, q.y, acc.pr));
             acc.pi = fma(P[i].x, q.y, fma(-P[i].y, q.x, acc.pi));
         }
@@ -207,7 +210,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
 // SAVE: some instance stores states (partial runs); SNAP: some instance takes snapshots.
 // Both are off for plain batches, which then carry neither store in the step loop.
 template <bool SAVE, bool SNAP>
-__global__ void __launch_bounds__(64 * TDSE_MAX_WAVES)
+__global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_WAVES_PER_SIMD)
 tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst, int *counter,
                       double2 *work, long long work_stride)
 {

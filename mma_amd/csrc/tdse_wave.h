@@ -17,6 +17,25 @@ __device__ inline double2 shfl_down2(double2 v, int d)
 {
     return make_double2(__shfl_down(v.x, d, 64), __shfl_down(v.y, d, 64));
 }
+// Loads and stores through global-address-space pointers: the arrays live in device memory
+// but arrive through structs, where the compiler would otherwise emit flat accesses (which
+// wait on both the memory and the LDS counters).
+#define TDSE_GLOBAL __attribute__((address_space(1)))
+typedef double tdse_d2 __attribute__((ext_vector_type(2)));
+__device__ inline double2 ld2(const double2 *p, long long i)
+{
+    const tdse_d2 v = ((const TDSE_GLOBAL tdse_d2 *)p)[i];
+    return make_double2(v.x, v.y);
+}
+__device__ inline void st2(double2 *p, long long i, double2 v)
+{
+    tdse_d2 w;
+    w.x = v.x;
+    w.y = v.y;
+    ((TDSE_GLOBAL tdse_d2 *)p)[i] = w;
+}
+__device__ inline double ld1(const double *p, long long i) { return ((const TDSE_GLOBAL double *)p)[i]; }
+
 __device__ inline double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
