@@ -165,7 +165,7 @@ int build_tables(ctdse_geometry *G)
                                                      " too close to 1 for the segmented solve");
     const int W = (int)std::ceil(std::log(kTruncTol) / std::log(rho));
     const int H = (W + TDSE_CHUNK - 1) / TDSE_CHUNK;
-    if (H > TDSE_MAX_HALO || TDSE_MAX_WAVES * 64 - 2 * H < 64)
+    if (H > TDSE_MAX_HALO || TDSE_MAX_WAVES * 64 - H < 64)
         return set_status(CTDSE_ERR_UNSUPPORTED, "truncation depth " + std::to_string(W) +
                                                      " points exceeds the segment size (dt/dx^2 too large)");
     if (J.pint_lo < 0 || J.pint_hi > G->num_r)
@@ -266,7 +266,7 @@ struct InstSpec {
 int pick_waves(int *min_out, int max_halo_lanes)
 {
     int min_nw = 1;
-    while (min_nw * 64 - 2 * max_halo_lanes < 64) min_nw++;
+    while (min_nw * 64 - max_halo_lanes < 64) min_nw++;
     int nw = std::max(4, min_nw);
     if (const char *env = std::getenv("MMA_TDSE_WAVES")) nw = std::max(min_nw, std::min(TDSE_MAX_WAVES, std::atoi(env)));
     *min_out = min_nw;

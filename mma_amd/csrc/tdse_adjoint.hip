@@ -12,7 +12,8 @@
 //     forward   y_j  = mu_j + conj(h_{j-1}) y_{j-1},     h_{j-1} = -(1 + i g_j) ib_{j-1}
 //     backward  nu_j = conj(ib_j) y_j + conj(p_j) nu_{j+1}, p_j  = -(1 + i g_j) ib_j
 // two first-order recurrences with the forward's decay rate, evaluated with the same
-// segmented scan and truncation as the forward solve.  A'^H = M2' + i H'^T with
+// segmented scan as the forward solve: y chained exactly from segment to segment, nu
+// truncated H chunks right of each segment's own chunks.  A'^H = M2' + i H'^T with
 // (H'^T v)_j = g_j (v_{j-1} + v_{j+1}) + d_j v_j.
 #include "tdse_internal.h"
 #include "tdse_wave.h"
@@ -21,45 +22,48 @@ namespace {
 
 using namespace tdse;
 
+// edge_in / edge_out: {old kappa, y} at the last own point of the previous / this segment.
 __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
                                 double2 *__restrict__ kap, const double2 *__restrict__ psik, double gobs, double Fdt,
                                 double2 rot_conj, double &gacc, double2 (*slot_f)[2], double2 (*slot_b)[2],
-                                double2 *halo_in, double2 *halo_out)
+                                const double2 *edge_in, double2 *edge_out)
 {
-    const int H = G.halo_lanes;
     const int gl = wave * 64 + lane;
-    const int nl = nw * 64;
-    const int c = seg * own_chunks - H + gl;
-    const int cl = min(max(c, -1), G.nchunks);
+    const int c0 = seg * own_chunks;
+    const int nwl = min(nw, (G.nchunks - c0 + 63) >> 6);  // waves with a lane on the grid
+    if (wave >= nwl) {
+        __syncthreads();  // the two barriers of block_carry
+        __syncthreads();
+        return;
+    }
+    const int nl = nwl * 64;
+    const int c = c0 + gl;
+    const int cl = min(c, G.nchunks);
     const long long base = tdse_chunk_base(cl);
-    const bool from_lds = seg > 0 && gl < H;
-    const bool save_old = gl >= own_chunks && gl < own_chunks + H;
 
     double2 P[C], IB[C];
     double Gv[C];
 #pragma unroll
     for (int i = 0; i < C; i++) {
-        P[i] = from_lds ? halo_in[gl * C + i] : ld2(kap, base + 64 * i);
+        P[i] = ld2(kap, base + 64 * i);
         Gv[i] = ld1(G.g, base + 64 * i);
         IB[i] = ld2(G.ib, base + 64 * i);
     }
-    if (save_old) {
-#pragma unroll
-        for (int i = 0; i < C; i++) halo_out[(gl - own_chunks) * C + i] = P[i];
-    }
+    if (gl == own_chunks - 1) edge_out[0] = P[C - 1];
     const double2 zero2 = make_double2(0.0, 0.0);
     double2 left = dpp2<kWaveShr1>(P[C - 1], zero2), right = dpp2<kWaveShl1>(P[0], zero2);
     double2 ibL = dpp2<kWaveShr1>(IB[C - 1], zero2);
     if (lane == 0) {
-        const long long nb = tdse_chunk_base(min(max(c - 1, -1), G.nchunks)) + 64 * (C - 1);
-        left = gl > 0 ? ld2(kap, nb) : zero2;  // gl >= 64 > H: not yet overwritten
-        ibL = gl > 0 ? ld2(G.ib, nb) : zero2;
+        const long long nb = tdse_chunk_base(min(c - 1, G.nchunks)) + 64 * (C - 1);  // chunk -1: zero padding
+        left = gl > 0 ? ld2(kap, nb) : (seg > 0 ? edge_in[0] : zero2);  // gl > 0: not yet overwritten
+        ibL = ld2(G.ib, nb);
     }
     if (lane == 63) {
-        const long long nb = tdse_chunk_base(min(max(c + 1, -1), G.nchunks));
+        const long long nb = tdse_chunk_base(min(c + 1, G.nchunks));
         right = gl < nl - 1 ? ld2(kap, nb) : zero2;
     }
-    const bool own = gl >= H && gl < nl - H && c < G.nchunks;
+    const double2 y_in = seg > 0 ? edge_in[1] : zero2;
+    const bool own = gl < own_chunks && c < G.nchunks;
 
     // lambda = A'^H kappa + source, the gradient term, then the conjugate phase
     {
@@ -106,7 +110,7 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
         }
         double2 Q = ld2(G.adq, cl + 64);
         wave_scan<true>(Q, y, lane);
-        y = block_carry<true>(Q, y, wave, lane, nw, slot_f);
+        y = block_carry<true>(Q, y, wave, lane, nwl, slot_f, y_in);
 #pragma unroll
         for (int i = 0; i < C; i++) {
             const double2 b = (i > 0) ? IB[i - 1] : ibL;
@@ -115,6 +119,7 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
             y = make_double2(P[i].x + hr * y.x - hi * y.y, P[i].y + hr * y.y + hi * y.x);
             P[i] = y;
         }
+        if (gl == own_chunks - 1) edge_out[1] = y;
     }
 
     // backward recurrence nu_j = conj(ib_j) y_j + conj(p_j) nu_{j+1}
@@ -130,7 +135,7 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
         }
         double2 Gt = ld2(G.adg, cl + 64);
         wave_scan<false>(Gt, u, lane);
-        u = block_carry<false>(Gt, u, wave, lane, nw, slot_b);
+        u = block_carry<false>(Gt, u, wave, lane, nwl, slot_b);
 #pragma unroll
         for (int i = C - 1; i >= 0; i--) {
             const double2 ib = IB[i];
@@ -155,7 +160,7 @@ tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *insts, const int *
 {
     __shared__ double part[2][TDSE_MAX_WAVES];
     __shared__ double2 slot_f[TDSE_MAX_WAVES][2], slot_b[TDSE_MAX_WAVES][2];
-    __shared__ double2 halo[2][TDSE_MAX_HALO * TDSE_CHUNK];
+    __shared__ double2 edge[2][2];
     __shared__ int cur;
     const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
@@ -163,13 +168,13 @@ tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *insts, const int *
     for (;;) {
         if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
         __syncthreads();
-        const int t = cur;
+        const int t = __builtin_amdgcn_readfirstlane(cur);  // uniform: keeps the instance in SGPRs
         __syncthreads();
         if (t >= n_inst) break;
         const TdseAdjInst &I = insts[order[t]];
         const TdseGeom &G = geoms[I.geom];
         const long long npad = G.npad;
-        const int own_chunks = nw * 64 - 2 * G.halo_lanes;
+        const int own_chunks = nw * 64 - G.halo_lanes;
         const int nseg = (G.nchunks + own_chunks - 1) / own_chunks;
         for (int k = I.k_hi; k > I.k_lo; k--) {
             const double F = I.field[k - 1];
@@ -182,7 +187,7 @@ tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *insts, const int *
             double gacc = 0.0;
             for (int s = 0; s < nseg; s++)
                 adjoint_segment(G, s, own_chunks, wave, lane, nw, I.kappa, psik, gobs, Fdt, rot_conj, gacc, slot_f,
-                                slot_b, halo[s & 1], halo[(s + 1) & 1]);
+                                slot_b, edge[s & 1], edge[(s + 1) & 1]);
             const double gw = wave_sum(gacc);
             if (lane == 0) part[k & 1][wave] = gw;
             __syncthreads();

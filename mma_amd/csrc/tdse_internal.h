@@ -21,7 +21,7 @@
 #define TDSE_WAVES_PER_SIMD 3      // propagation kernel: register budget 168 VGPRs, 12 waves per CU
 #define TDSE_WAVES_PER_CU (4 * TDSE_WAVES_PER_SIMD)
 #define TDSE_ADJ_WAVES_PER_CU 8    // adjoint kernel: 2 waves per SIMD (224 VGPRs)
-#define TDSE_MAX_HALO 40  // largest truncation depth in chunks (halo saved in LDS per segment)  // occupancy of the propagation kernel (2 waves per SIMD)
+#define TDSE_MAX_HALO 40           // largest truncation depth in chunks
 
 // storage offset of chunk c (c >= -64), row i of the chunk
 __host__ __device__ inline long long tdse_chunk_base(long long c) { return ((c >> 6) + 1) * TDSE_TILE + (c & 63); }
@@ -53,7 +53,7 @@ struct TdseGeom {
     int nx;               // num_r + 1 grid points
     int nchunks;          // ceil(nx / TDSE_CHUNK)
     long long npad;       // tiled array length, tdse_tiled_size(nchunks)
-    int halo_lanes;       // H: truncation depth in chunks on each end of a workgroup segment
+    int halo_lanes;       // H: truncation depth in chunks at the right end of a workgroup segment
     int pint_lo, pint_hi; // PopInt window [k1, k4] (prop.c:311-318)
     int p0_lo, p0_hi;     // rows where |psi0| > 1e-30 max|psi0|
     double dx, dt, xmax;  // x_j = j dx - xmax

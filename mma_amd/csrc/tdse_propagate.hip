@@ -2,23 +2,26 @@
 //
 // One workgroup of NW waves owns one instance at a time (instances are taken from
 // an atomic counter in the given order, largest first) and runs all its time
-// steps.  Per step, the grid is covered by segments; one wave solves one segment:
+// steps.  Per step, the grid is covered by segments, solved in order by all its waves:
 //
-//   lanes 0..63 each own one chunk of TDSE_CHUNK consecutive points of the
-//   segment's extended window, which is the segment's own chunks plus H halo
-//   chunks on each side.  The tridiagonal solve B'u = A'psi uses the global
+//   lanes each own one chunk of TDSE_CHUNK consecutive points; a segment's window is its
+//   own chunks followed by H halo chunks.  The tridiagonal solve B'u = A'psi uses the global
 //   Thomas factorisation of B' (pivot table ib, step-independent), written as two
 //   first-order linear recurrences
 //       forward   z_j = ib_j (r_j - a'_j z_{j-1})
 //       backward  u_j = z_j + h_j u_{j+1},   h_j = -c'_j ib_j
 //   each evaluated in two passes around a 64-lane scan of chunk transfer maps
 //   (carry-out with zero carry-in, exclusive scan with the chunk products from
-//   the tables, recompute with the true carry-in).  The two recurrences are
-//   started from zero at the ends of the extended window: the inverse of B' decays
-//   by |q|, |h| <= rho < 1 per point, so after H*TDSE_CHUNK >= log(1e-17)/log(rho)
-//   points the truncation is below 1e-17 of max|psi| and the own chunks are exact
-//   to rounding (truncated-SPIKE).  Segments of one step are independent, read
-//   psi of step k from one buffer and write step k+1 to the other.
+//   the tables, recompute with the true carry-in).  The forward recurrence is exact: it
+//   enters each segment with z at the last point of the previous segment, which that
+//   segment left in LDS (segments run left to right).  The backward recurrence starts
+//   from zero at the right end of the window: the inverse of B' decays by |h| <= rho < 1
+//   per point, so after H*TDSE_CHUNK >= log(1e-17)/log(rho) points the truncation is
+//   below 1e-17 of max|psi| and the own chunks are exact to rounding (truncated SPIKE).
+//   Segments read psi of step k and overwrite it with step k+1 in place: a segment reads
+//   no chunk an earlier one wrote except the point just left of its window, whose old
+//   value the previous segment also left in LDS.  Waves whose lanes all lie past the
+//   grid (the tail of the last segment) skip the solve and only join the barriers.
 //
 // Observables (prop.c:293-347) are reduced per wave, then across waves in LDS in a
 // fixed order (deterministic).  HBM traffic per grid-point-step: psi read + write,
@@ -34,55 +37,58 @@ using namespace tdse;
 // snap_row != nullptr: also store the new psi there (natural order).
 // Each lane keeps its chunk of psi and the chunk's coefficients (g_j, 1/beta_j) in
 // registers for the whole solve: all loads of a segment are issued together, once.
-// psi is updated in place: the segments of a step are solved in order, and the only
-// chunks a segment reads after the previous segment has overwritten them are its H
-// left-halo chunks, whose old values the previous segment saved to LDS (halo_old).
+// edge_in / edge_out: {old psi, forward z} at the last own point of the previous / this
+// segment (LDS, alternating between segments).
 template <bool SAVE, bool SNAP>
 __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
                               double2 *__restrict__ psi, double Fdt, double2 rot, Acc &acc, double *snap_row,
-                              double2 (*slot_f)[2], double2 (*slot_b)[2], double2 *halo_in, double2 *halo_out,
+                              double2 (*slot_f)[2], double2 (*slot_b)[2], const double2 *edge_in, double2 *edge_out,
                               double2 *__restrict__ save_to)
 {
-    const int H = G.halo_lanes;
     const int gl = wave * 64 + lane;                 // lane index in the workgroup
-    const int nl = nw * 64;
-    const int c = seg * own_chunks - H + gl;
-    // Lanes of the first and last segment reach past the grid; they read chunk -1 or
-    // chunk nchunks instead, which are zero in every buffer and table (padding rows).
-    const int cl = min(max(c, -1), G.nchunks);
+    const int cs = seg * own_chunks;                 // first own chunk
+    // live waves: those with a lane on the grid (chunks past the grid are zero padding and
+    // carry the zero map, so dropping whole waves of them changes nothing)
+    const int nwl = min(nw, (G.nchunks - cs + 63) >> 6);
+    if (wave >= nwl) {
+        __syncthreads();  // the two barriers of block_carry
+        __syncthreads();
+        return;
+    }
+    const int nl = nwl * 64;
+    const int c = cs + gl;
+    // lanes past the grid read chunk nchunks instead, which is zero in every buffer and
+    // table (padding row)
+    const int cl = min(c, G.nchunks);
     const long long base = tdse_chunk_base(cl);
 
     double2 P[C], IB[C];
     double Gv[C];
-    const bool from_lds = seg > 0 && gl < H;               // chunk already overwritten by segment seg-1
-    const bool save_old = gl >= own_chunks && gl < own_chunks + H;  // next segment's left halo
 #pragma unroll
     for (int i = 0; i < C; i++) {
-        P[i] = from_lds ? halo_in[gl * C + i] : ld2(psi, base + 64 * i);
+        P[i] = ld2(psi, base + 64 * i);
         Gv[i] = ld1(G.g, base + 64 * i);
         IB[i] = ld2(G.ib, base + 64 * i);
     }
-    if (save_old) {
-#pragma unroll
-        for (int i = 0; i < C; i++) halo_out[(gl - own_chunks) * C + i] = P[i];
-    }
+    if (gl == own_chunks - 1) edge_out[0] = P[C - 1];  // the next segment's left neighbour
     const double2 Qc = ld2(G.chq, cl + 64);
     const double2 Gc = ld2(G.chg, cl + 64);
     // neighbours across the chunk edges: shuffles inside a wave, loads across waves,
-    // zero outside the extended window (truncation zone)
+    // LDS for the point left of the window, zero past the window's right end
     const double2 zero2 = make_double2(0.0, 0.0);
     double2 left = dpp2<kWaveShr1>(P[C - 1], zero2), right = dpp2<kWaveShl1>(P[0], zero2);
     double gL = dpp<kWaveShr1>(Gv[C - 1], 0.0), gR = dpp<kWaveShl1>(Gv[0], 0.0);
     if (lane == 0) {
-        const long long nb = tdse_chunk_base(min(max(c - 1, -1), G.nchunks)) + 64 * (C - 1);
-        left = gl > 0 ? ld2(psi, nb) : make_double2(0.0, 0.0);  // gl >= 64 > H: not yet overwritten
-        gL = gl > 0 ? ld1(G.g, nb) : 0.0;
+        const long long nb = tdse_chunk_base(min(c - 1, G.nchunks)) + 64 * (C - 1);  // chunk -1: zero padding
+        left = gl > 0 ? ld2(psi, nb) : (seg > 0 ? edge_in[0] : zero2);  // gl > 0: not yet overwritten
+        gL = ld1(G.g, nb);
     }
     if (lane == 63) {
-        const long long nb = tdse_chunk_base(min(max(c + 1, -1), G.nchunks));
-        right = gl < nl - 1 ? ld2(psi, nb) : make_double2(0.0, 0.0);
+        const long long nb = tdse_chunk_base(min(c + 1, G.nchunks));
+        right = gl < nl - 1 ? ld2(psi, nb) : zero2;
         gR = gl < nl - 1 ? ld1(G.g, nb) : 0.0;
     }
+    const double2 z_in = seg > 0 ? edge_in[1] : zero2;
 
     // r' = (M2' - i H') psi
     {
@@ -117,7 +123,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
         }
         double2 Q = Qc;
         wave_scan<true>(Q, z, lane);
-        z = block_carry<true>(Q, z, wave, lane, nw, slot_f);
+        z = block_carry<true>(Q, z, wave, lane, nwl, slot_f, z_in);
 #pragma unroll
         for (int i = 0; i < C; i++) {
             const double gm = (i > 0) ? Gv[i - 1] : gL;
@@ -126,6 +132,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
             z = cmul(make_double2(tr, ti), IB[i]);
             P[i] = z;
         }
+        if (gl == own_chunks - 1) edge_out[1] = z;  // the next segment's forward carry
     }
 
     // backward recurrence u_j = z_j + h_j u_{j+1},  h_j = -(1 + i g_{j+1}) ib_j
@@ -139,7 +146,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
         }
         double2 Gt = Gc;
         wave_scan<false>(Gt, u, lane);
-        u = block_carry<false>(Gt, u, wave, lane, nw, slot_b);
+        u = block_carry<false>(Gt, u, wave, lane, nwl, slot_b);
 #pragma unroll
         for (int i = C - 1; i >= 0; i--) {
             const double gp = (i + 1 < C) ? Gv[i + 1] : gR;
@@ -152,7 +159,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     // keep the loads of the observable tables below this point: hoisted above the passes they
     // would hold registers through the whole solve
     asm volatile("" ::: "memory");
-    if (gl < H || gl >= nl - H || c >= G.nchunks) return;  // only the segment's own chunks continue
+    if (gl >= own_chunks || c >= G.nchunks) return;  // only the segment's own chunks continue
 
     // length-gauge phase exp(+i F dt x_j) (prop.c:199-204) by rotation from the chunk start,
     // store, and the observables of prop.c:302-333 that need every point
@@ -216,7 +223,7 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
 {
     __shared__ double part[2][TDSE_MAX_WAVES][5];
     __shared__ double2 slot_f[TDSE_MAX_WAVES][2], slot_b[TDSE_MAX_WAVES][2];
-    __shared__ double2 halo[2][TDSE_MAX_HALO * TDSE_CHUNK];
+    __shared__ double2 edge[2][2];
     __shared__ int cur;
     const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
@@ -226,13 +233,13 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
     for (;;) {
         if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
         __syncthreads();
-        const int t = cur;
+        const int t = __builtin_amdgcn_readfirstlane(cur);  // uniform: keeps the instance in SGPRs
         __syncthreads();
         if (t >= n_inst) break;
         const TdseInst &I = insts[order[t]];
         const TdseGeom &G = geoms[I.geom];
         const long long npad = G.npad;
-        const int own_chunks = nw * 64 - 2 * G.halo_lanes;  // host guarantees > 0
+        const int own_chunks = nw * 64 - G.halo_lanes;  // host guarantees >= 64
         const int nseg = (G.nchunks + own_chunks - 1) / own_chunks;
 
         // start state: psi0 (t = tmin) or a saved state; padding chunks are zero in both
@@ -275,8 +282,8 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
             }
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
             for (int s = 0; s < nseg; s++)
-                solve_segment<SAVE, SNAP>(G, s, own_chunks, wave, lane, nw, buf, Fdt, rot, acc, snap_row, slot_f, slot_b,
-                              halo[s & 1], halo[(s + 1) & 1], save_to);
+                solve_segment<SAVE, SNAP>(G, s, own_chunks, wave, lane, nw, buf, Fdt, rot, acc, snap_row, slot_f,
+                                          slot_b, edge[s & 1], edge[(s + 1) & 1], save_to);
             // a final snapshot that coincides with the last step is written after the loop
             const double pr = wave_sum(acc.pr), pi = wave_sum(acc.pi), pint = wave_sum(acc.pint);
             const double pos = wave_sum(acc.pos), gp = wave_sum(acc.gp);

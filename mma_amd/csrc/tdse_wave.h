@@ -138,9 +138,12 @@ __device__ inline double wave_sum(double v)
 // Cross-wave part of an exclusive scan of affine maps x_out = z + Q x_in.
 // (Qi, zi): the wave-inclusive composition of lanes [0..lane] (forward) or [lane..63]
 // (backward).  Returns the carry into this lane from everything before it in the
-// workgroup-wide lane order (zero at the start of the extended window).
+// workgroup-wide lane order over the nw live waves.  The value entering the window is
+// x_in (forward: the exact value before lane 0 of wave 0; backward: zero after the last
+// live wave).
 template <bool FWD>
-__device__ inline double2 block_carry(double2 Qi, double2 zi, int wave, int lane, int nw, double2 (*slot)[2])
+__device__ inline double2 block_carry(double2 Qi, double2 zi, int wave, int lane, int nw, double2 (*slot)[2],
+                                      double2 x_in = make_double2(0.0, 0.0))
 {
     // exclusive within the wave: shift the inclusive maps by one lane
     const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
@@ -152,7 +155,7 @@ __device__ inline double2 block_carry(double2 Qi, double2 zi, int wave, int lane
     }
     __syncthreads();
     // carry into this wave: compose the totals of the waves before it
-    double2 Z = make_double2(0.0, 0.0);
+    double2 Z = FWD ? x_in : make_double2(0.0, 0.0);
     if (FWD) {
         for (int v = 0; v < wave; v++) {
             const double2 q = slot[v][0], z = slot[v][1];
