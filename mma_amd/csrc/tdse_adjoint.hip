@@ -32,8 +32,8 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
     const int c0 = seg * own_chunks;
     const int nwl = min(nw, (G.nchunks - c0 + 63) >> 6);  // waves with a lane on the grid
     if (wave >= nwl) {
-        __syncthreads();  // the two barriers of block_carry
-        __syncthreads();
+        lds_barrier();  // the two barriers of block_carry
+        lds_barrier();
         return;
     }
     const int nl = nwl * 64;

@@ -23,8 +23,9 @@
 //   value the previous segment also left in LDS.  Waves whose lanes all lie past the
 //   grid (the tail of the last segment) skip the solve and only join the barriers.
 //
-// Observables (prop.c:293-347) are reduced per wave, then across waves in LDS in a
-// fixed order (deterministic).  HBM traffic per grid-point-step: psi read + write,
+// Observables (prop.c:293-347) of the state after step k are taken in step k + 1 from the
+// psi it loads anyway (a final pass covers the last state of a partial run); they are
+// reduced per wave, then across waves in LDS in a fixed order (deterministic).  HBM traffic per grid-point-step: psi read + write,
 // 32 bytes; the tables are shared by all instances of a geometry and stay in L2.
 #include "tdse_internal.h"
 #include "tdse_wave.h"
@@ -33,7 +34,69 @@ namespace {
 
 using namespace tdse;
 
+// Observables of prop.c:302-333 that need every point, over one lane's chunk of a state
+// (P, the chunk's rows at tiled offset base, first point j0): sum |psi|^2 x, sum |psi|^2 V',
+// <psi0|psi> over the rows where psi0 is not negligible, and the PopInt window [k1, k4]
+// (dp: V' at the chunk's rows).
+// Only lanes holding one of the segment's own chunks add (own).
+__device__ inline void observe_chunk(const TdseGeom &G, const double2 (&P)[C], const double (&dp)[C], long long base,
+                                     int j0, bool own, Acc &acc)
+{
+    if (!own) return;
+    const bool in_pint = j0 + C - 1 >= G.pint_lo && j0 <= G.pint_hi;
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const double2 v = P[i];
+        const double n2 = fma(v.x, v.x, v.y * v.y);
+        const double xj = __dsub_rn(__dmul_rn((double)(j0 + i), G.dx), G.xmax);
+        acc.pos = fma(n2, xj, acc.pos);
+        acc.gp = fma(n2, dp[i], acc.gp);
+        const int j = j0 + i;
+        acc.pint = fma(in_pint && j >= G.pint_lo && j <= G.pint_hi ? 1.0 : 0.0, n2, acc.pint);  // exact 0 or n2
+    }
+    if (j0 + C - 1 >= G.p0_lo && j0 <= G.p0_hi) {
+#pragma unroll
+        for (int i = 0; i < C; i++) {
+            const double2 q = ld2(G.psi0, base + 64 * i);
+            acc.pr = fma(P[i].x, q.x, fma(P[i].y, q.y, acc.pr));
+            acc.pi = fma(P[i].x, q.y, fma(-P[i].y, q.x, acc.pi));
+        }
+    }
+}
+
+// Step-independent table values of one lane's chunk in one segment.
+struct SegTables {
+    double2 IB[C];   // 1/beta_j
+    double Gv[C];    // g_j
+    double dp[C];    // V'(x_j)
+    double2 Qc, Gc;  // chunk transfer maps (forward, backward)
+    double gL, gR;   // g left of the chunk (lane 0) and right of it (lane 63)
+};
+
+__device__ inline void load_tables(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
+                                   SegTables &T)
+{
+    const int cs = seg * own_chunks;
+    const int nwl = min(nw, (G.nchunks - cs + 63) >> 6);
+    const int gl = wave * 64 + lane;
+    const int c = cs + gl;
+    const int cl = min(c, G.nchunks);
+    const long long base = tdse_chunk_base(cl);
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        T.Gv[i] = ld1(G.g, base + 64 * i);
+        T.IB[i] = ld2(G.ib, base + 64 * i);
+        T.dp[i] = ld1(G.dpot, base + 64 * i);
+    }
+    T.Qc = ld2(G.chq, cl + 64);
+    T.Gc = ld2(G.chg, cl + 64);
+    T.gL = lane == 0 ? ld1(G.g, tdse_chunk_base(min(c - 1, G.nchunks)) + 64 * (C - 1)) : 0.0;  // chunk -1: zero
+    T.gR = lane == 63 && gl < nwl * 64 - 1 ? ld1(G.g, tdse_chunk_base(min(c + 1, G.nchunks))) : 0.0;
+}
+
 // Solve one segment of one step with all waves of the workgroup.
+// observe: add the observables of the state the step starts from (psi_k, read for the
+// solve anyway) to acc; the step's own result is observed by the next step.
 // snap_row != nullptr: also store the new psi there (natural order).
 // Each lane keeps its chunk of psi and the chunk's coefficients (g_j, 1/beta_j) in
 // registers for the whole solve: all loads of a segment are issued together, once.
@@ -41,9 +104,9 @@ using namespace tdse;
 // segment (LDS, alternating between segments).
 template <bool SAVE, bool SNAP>
 __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
-                              double2 *__restrict__ psi, double Fdt, double2 rot, Acc &acc, double *snap_row,
-                              double2 (*slot_f)[2], double2 (*slot_b)[2], const double2 *edge_in, double2 *edge_out,
-                              double2 *__restrict__ save_to)
+                              double2 *__restrict__ psi, double2 e0, double2 rot, bool observe, Acc &acc,
+                              double *snap_row, double2 (*slot_f)[2], double2 (*slot_b)[2], const double2 *edge_in,
+                              double2 *edge_out, double2 *__restrict__ save_to)
 {
     const int gl = wave * 64 + lane;                 // lane index in the workgroup
     const int cs = seg * own_chunks;                 // first own chunk
@@ -51,8 +114,8 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     // carry the zero map, so dropping whole waves of them changes nothing)
     const int nwl = min(nw, (G.nchunks - cs + 63) >> 6);
     if (wave >= nwl) {
-        __syncthreads();  // the two barriers of block_carry
-        __syncthreads();
+        lds_barrier();  // the two barriers of block_carry
+        lds_barrier();
         return;
     }
     const int nl = nwl * 64;
@@ -62,32 +125,34 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     const int cl = min(c, G.nchunks);
     const long long base = tdse_chunk_base(cl);
 
-    double2 P[C], IB[C];
-    double Gv[C];
+    SegTables T;
+    load_tables(G, seg, own_chunks, wave, lane, nw, T);
+    double2 P[C];
 #pragma unroll
-    for (int i = 0; i < C; i++) {
-        P[i] = ld2(psi, base + 64 * i);
-        Gv[i] = ld1(G.g, base + 64 * i);
-        IB[i] = ld2(G.ib, base + 64 * i);
-    }
-    if (gl == own_chunks - 1) edge_out[0] = P[C - 1];  // the next segment's left neighbour
-    const double2 Qc = ld2(G.chq, cl + 64);
-    const double2 Gc = ld2(G.chg, cl + 64);
-    // neighbours across the chunk edges: shuffles inside a wave, loads across waves,
-    // LDS for the point left of the window, zero past the window's right end
+    for (int i = 0; i < C; i++) P[i] = ld2(psi, base + 64 * i);
     const double2 zero2 = make_double2(0.0, 0.0);
+    double2 nb = zero2;  // lane 0: the point left of the wave's first chunk; lane 63: right of its last
+    if (lane == 0 && gl > 0) nb = ld2(psi, tdse_chunk_base(c - 1) + 64 * (C - 1));  // not yet overwritten
+    if (lane == 63 && gl < nl - 1) nb = ld2(psi, tdse_chunk_base(min(c + 1, G.nchunks)));
+    const bool own = gl < own_chunks && c < G.nchunks;
+    const int j0 = c * C;
+    // observables of the state this step starts from (psi_k)
+    if (observe) observe_chunk(G, P, T.dp, base, j0, own, acc);
+    const double2(&IB)[C] = T.IB;
+    const double(&Gv)[C] = T.Gv;
+    const double2 Qc = T.Qc, Gc = T.Gc;
+    double gL = T.gL, gR = T.gR;
+    if (gl == own_chunks - 1) edge_out[0] = P[C - 1];  // the next segment's left neighbour
+    // neighbours across the chunk edges: shuffles inside a wave, loads across waves, LDS for
+    // the point left of the window, zero past the window's right end
+    {
+        const double gLd = dpp<kWaveShr1>(Gv[C - 1], 0.0), gRd = dpp<kWaveShl1>(Gv[0], 0.0);
+        gL = lane == 0 ? gL : gLd;
+        gR = lane == 63 ? gR : gRd;
+    }
     double2 left = dpp2<kWaveShr1>(P[C - 1], zero2), right = dpp2<kWaveShl1>(P[0], zero2);
-    double gL = dpp<kWaveShr1>(Gv[C - 1], 0.0), gR = dpp<kWaveShl1>(Gv[0], 0.0);
-    if (lane == 0) {
-        const long long nb = tdse_chunk_base(min(c - 1, G.nchunks)) + 64 * (C - 1);  // chunk -1: zero padding
-        left = gl > 0 ? ld2(psi, nb) : (seg > 0 ? edge_in[0] : zero2);  // gl > 0: not yet overwritten
-        gL = ld1(G.g, nb);
-    }
-    if (lane == 63) {
-        const long long nb = tdse_chunk_base(min(c + 1, G.nchunks));
-        right = gl < nl - 1 ? ld2(psi, nb) : zero2;
-        gR = gl < nl - 1 ? ld1(G.g, nb) : 0.0;
-    }
+    if (lane == 0) left = gl > 0 ? nb : (seg > 0 ? edge_in[0] : zero2);
+    if (lane == 63) right = nb;
     const double2 z_in = seg > 0 ? edge_in[1] : zero2;
 
     // r' = (M2' - i H') psi
@@ -156,19 +221,11 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
         }
     }
 
-    // keep the loads of the observable tables below this point: hoisted above the passes they
-    // would hold registers through the whole solve
-    asm volatile("" ::: "memory");
-    if (gl >= own_chunks || c >= G.nchunks) return;  // only the segment's own chunks continue
+    if (!own) return;  // only the segment's own chunks continue
 
-    // length-gauge phase exp(+i F dt x_j) (prop.c:199-204) by rotation from the chunk start,
-    // store, and the observables of prop.c:302-333 that need every point
-    const int j0 = c * C;
-    const double x0 = __dsub_rn(__dmul_rn((double)j0, G.dx), G.xmax);
-    double s0, c0;
-    sincos(Fdt * x0, &s0, &c0);
-    double2 e = make_double2(c0, s0);
-    const double *dp = G.dpot + base;
+    // length-gauge phase exp(+i F dt x_j) (prop.c:199-204) by rotation from the chunk start
+    // (e0), then store
+    double2 e = e0;
 #pragma unroll
     for (int i = 0; i < C; i++) {
         const double2 v = cmul(e, P[i]);
@@ -177,29 +234,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
         if constexpr (SAVE) {
             if (save_to) st2(save_to, base + 64 * i, v);
         }
-        const double n2 = fma(v.x, v.x, v.y * v.y);
-        const double xj = __dsub_rn(__dmul_rn((double)(j0 + i), G.dx), G.xmax);
-        acc.pos = fma(n2, xj, acc.pos);
-        acc.gp = fma(n2, ld1(dp, 64 * i), acc.gp);
         e = cmul(e, rot);
-    }
-    // <psi0|psi> over the rows where psi0 is not negligible
-    if (j0 + C - 1 >= G.p0_lo && j0 <= G.p0_hi) {
-        const double2 *q0 = G.psi0 + base;
-#pragma unroll
-        for (int i = 0; i < C; i++) {
-            const double2 q = ld2(q0, 64 * i);
-            acc.pr = fma(P[i].x, q.x, fma(P[i].y, q.y, acc.pr));
-            acc.pi = fma(P[i].x, q.y, fma(-P[i].y, q.x, acc.pi));
-        }
-    }
-    // PopInt window [k1, k4]
-    if (j0 + C - 1 >= G.pint_lo && j0 <= G.pint_hi) {
-#pragma unroll
-        for (int i = 0; i < C; i++) {
-            const int j = j0 + i;
-            if (j >= G.pint_lo && j <= G.pint_hi) acc.pint += fma(P[i].x, P[i].x, P[i].y * P[i].y);
-        }
     }
     if (SNAP && snap_row) {
 #pragma unroll
@@ -212,14 +247,68 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     }
 }
 
+// Observables of a whole state in memory (psi, tiled), with the lane-to-chunk assignment
+// of the solve: the final state of a partial run, which no later step observes.
+__device__ inline void observe_state(const TdseGeom &G, int nseg, int own_chunks, int wave, int lane, int nw,
+                                     const double2 *psi, Acc &acc)
+{
+    for (int s = 0; s < nseg; s++) {
+        const int cs = s * own_chunks;
+        if (wave >= min(nw, (G.nchunks - cs + 63) >> 6)) continue;
+        const int c = cs + wave * 64 + lane;
+        const long long base = tdse_chunk_base(min(c, G.nchunks));
+        double2 P[C];
+        double dp[C];
+#pragma unroll
+        for (int i = 0; i < C; i++) {
+            P[i] = ld2(psi, base + 64 * i);
+            dp[i] = ld1(G.dpot, base + 64 * i);
+        }
+        observe_chunk(G, P, dp, base, c * C, wave * 64 + lane < own_chunks && c < G.nchunks, acc);
+    }
+}
+
+// Per-wave sums of the observables into part[wave] (lane 0 writes).
+__device__ inline void wave_observables(const Acc &acc, int wave, int lane, double (*part)[5])
+{
+    const double pr = wave_sum(acc.pr), pi = wave_sum(acc.pi), pint = wave_sum(acc.pint);
+    const double pos = wave_sum(acc.pos), gp = wave_sum(acc.gp);
+    if (lane == 0) {
+        part[wave][0] = pr;
+        part[wave][1] = pi;
+        part[wave][2] = pint;
+        part[wave][3] = pos;
+        part[wave][4] = gp;
+    }
+}
+
+// Entry idx of the outputs (prop.c:302-340) from the wave sums, added in wave order; F is
+// the field of the step that produced the state.  One thread.
+__device__ inline void store_observables(const TdseInst &I, const double (*part)[5], int nw, int idx, double F)
+{
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+    for (int w = 0; w < nw; w++) {
+        s0 += part[w][0];
+        s1 += part[w][1];
+        s2 += part[w][2];
+        s3 += part[w][3];
+        s4 += part[w][4];
+    }
+    I.src[idx] = -s4 + F;
+    I.ptot[idx] = s0 * s0 + s1 * s1;
+    I.pint[idx] = s2;
+    I.expv[idx] = s3;
+    if (I.efield) I.efield[idx] = F;
+}
+
 } // namespace
 
 // SAVE: some instance stores states (partial runs); SNAP: some instance takes snapshots.
 // Both are off for plain batches, which then carry neither store in the step loop.
 template <bool SAVE, bool SNAP>
 __global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_WAVES_PER_SIMD)
-tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst, int *counter,
-                      double2 *work, long long work_stride)
+tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
+                      const int *__restrict__ order, int n_inst, int *counter, double2 *work, long long work_stride)
 {
     __shared__ double part[2][TDSE_MAX_WAVES][5];
     __shared__ double2 slot_f[TDSE_MAX_WAVES][2], slot_b[TDSE_MAX_WAVES][2];
@@ -258,6 +347,7 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
 
         bool gating = !I.pregated;  // prop.c:125-134
         int i_snap = 0;
+        double F_prev = 0.0;  // field of the previous step
         for (int k = k_start; k < k_stop; k++) {
             const double fk = I.field[k];
             double F;
@@ -274,42 +364,42 @@ tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *o
             double rs, rc;
             sincos(Fdt * G.dx, &rs, &rc);
             const double2 rot = make_double2(rc, rs);
+            // the phase at the first point of this lane's chunk in segment 0, and its rotation
+            // from one segment to the next (own_chunks * TDSE_CHUNK points)
+            double2 e_lane, r_seg;
+            {
+                const double x = __dsub_rn(__dmul_rn((double)((wave * 64 + lane) * C), G.dx), G.xmax);
+                sincos(Fdt * x, &e_lane.y, &e_lane.x);
+                sincos(Fdt * ((double)(own_chunks * C) * G.dx), &r_seg.y, &r_seg.x);
+            }
             // snapshots, prop.c:248-255: row i_snap gets psi after step k when k % every == 0
             double *snap_row = nullptr;
             if (SNAP && I.snap && k % I.snap_every == 0 && i_snap < I.n_snap - 1) {
                 snap_row = I.snap + (long long)i_snap * 2 * G.nx;
                 i_snap++;
             }
+            // observables of psi_k (entry k), except for the run's start state, whose entry is
+            // the initial values (k = 0) or was written by the run that ended there
+            const bool observe = k > k_start;
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int s = 0; s < nseg; s++)
-                solve_segment<SAVE, SNAP>(G, s, own_chunks, wave, lane, nw, buf, Fdt, rot, acc, snap_row, slot_f,
-                                          slot_b, edge[s & 1], edge[(s + 1) & 1], save_to);
-            // a final snapshot that coincides with the last step is written after the loop
-            const double pr = wave_sum(acc.pr), pi = wave_sum(acc.pi), pint = wave_sum(acc.pint);
-            const double pos = wave_sum(acc.pos), gp = wave_sum(acc.gp);
-            if (lane == 0) {
-                part[k & 1][wave][0] = pr;
-                part[k & 1][wave][1] = pi;
-                part[k & 1][wave][2] = pint;
-                part[k & 1][wave][3] = pos;
-                part[k & 1][wave][4] = gp;
+            double2 e0 = e_lane;
+            for (int s = 0; s < nseg; s++) {
+                solve_segment<SAVE, SNAP>(G, s, own_chunks, wave, lane, nw, buf, e0, rot, observe, acc, snap_row,
+                                          slot_f, slot_b, edge[s & 1], edge[(s + 1) & 1], save_to);
+                e0 = cmul(e0, r_seg);
             }
+            wave_observables(acc, wave, lane, part[k & 1]);
+            __syncthreads();  // also: psi of step k + 1 is complete in memory
+            if (threadIdx.x == 0 && I.src && observe) store_observables(I, part[k & 1], nw, k, F_prev);
+            F_prev = F;
+        }
+        // the final state of a partial run: entry k_stop (a full run has no entry n_steps)
+        if (k_stop > k_start && k_stop < I.n_steps) {
+            Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
+            observe_state(G, nseg, own_chunks, wave, lane, nw, buf, acc);
+            wave_observables(acc, wave, lane, part[k_stop & 1]);
             __syncthreads();
-            if (threadIdx.x == 0 && I.src && k + 1 < I.n_steps) {
-                double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
-                for (int w = 0; w < nw; w++) {
-                    s0 += part[k & 1][w][0];
-                    s1 += part[k & 1][w][1];
-                    s2 += part[k & 1][w][2];
-                    s3 += part[k & 1][w][3];
-                    s4 += part[k & 1][w][4];
-                }
-                I.src[k + 1] = -s4 + F;
-                I.ptot[k + 1] = s0 * s0 + s1 * s1;
-                I.pint[k + 1] = s2;
-                I.expv[k + 1] = s3;
-                if (I.efield) I.efield[k + 1] = F;
-            }
+            if (threadIdx.x == 0 && I.src) store_observables(I, part[k_stop & 1], nw, k_stop, F_prev);
         }
         // prop.c:256-262: after the last step, row n_snap-1 gets the final state if every
         // earlier row was used; the final state itself is optional output

@@ -36,6 +36,12 @@ __device__ inline void st2(double2 *p, long long i, double2 v)
 }
 __device__ inline double ld1(const double *p, long long i) { return ((const TDSE_GLOBAL double *)p)[i]; }
 
+// Workgroup barrier for exchanges through LDS only: waits for this wave's LDS accesses but
+// not for its global loads and stores (a segment's psi stores stay in flight across the next
+// segment's barriers).  Global memory written by one wave and read by another needs
+// __syncthreads() in between.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 __device__ inline double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -153,7 +159,7 @@ __device__ inline double2 block_carry(double2 Qi, double2 zi, int wave, int lane
         slot[wave][0] = Qi;
         slot[wave][1] = zi;
     }
-    __syncthreads();
+    lds_barrier();
     // carry into this wave: compose the totals of the waves before it
     double2 Z = FWD ? x_in : make_double2(0.0, 0.0);
     if (FWD) {
