@@ -92,7 +92,8 @@ def cpu_baseline(target_s=8.0):
 def adjoint_measure(n_inst, n_t):
     """Gradient workload: n_inst instances, Nx = 8001, dt = 0.1, n_t field samples; loss = harmonic
     yield over harmonics 21-31.  Times one plain forward propagation (no checkpoints), the
-    checkpointing forward of TDSEDipole and its backward (recompute + adjoint sweep)."""
+    checkpointing forward of TDSEDipole and its backward (states rebuilt next to the adjoint
+    sweep), and once the backward that recomputes and stores each checkpoint block instead."""
     import torch
     from mma_amd import adjoint as AD, tdse, workloads as W
     g = W.ADJOINT_GRID
@@ -129,9 +130,19 @@ def adjoint_measure(n_inst, n_t):
         runs.append((t_ck, t_bwd, bool(torch.isfinite(E.grad).all())))
         del E, s, loss
     t_ck, t_bwd, ok = runs[-1]
+    E = dev.clone().requires_grad_(True)
+    loss = AD.harmonic_yield(AD.TDSEDipole.apply(E, geo, n, None, "stored"), g["dt"], win).sum()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_stored = time.perf_counter() - t0
+    log("backward with stored blocks %.3f s" % t_stored)
+    del E, loss
     return {"instances": n_inst, "Nx": g["num_r"] + 1, "dt": g["dt"], "steps": n, "forward_s": t_fwd,
             "checkpointing_forward_s": t_ck, "backward_s": t_bwd, "adjoint_over_forward": t_bwd / t_fwd,
             "forward_plus_backward_over_forward": (t_ck + t_bwd) / t_fwd, "gradient_finite": ok,
+            "backward_mode": "reconstruct", "stored_blocks_backward_s": t_stored,
             "first_pass_s": list(runs[0][:2]), "checkpoint_every": int(math.ceil(math.sqrt(n)))}
 
 

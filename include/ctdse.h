@@ -227,6 +227,22 @@ int ctdse_adjoint_batch(int n_inst, const ctdse_geometry *const *geom, const int
                         const double *states, const long long *states_off, double *kappa, const long long *kappa_off,
                         double *grad, const long long *grad_off, void *stream);
 
+/*
+ * Adjoint sweep that rebuilds the states (no counterpart in the reference): for k = k_hi[i]
+ * down to 1 it writes grad[k-1] = dL/dF_{k-1} (grad + grad_off[i]), given gobs[k] =
+ * dL/d sourceterm_k and the gated field F, with the adjoint state starting from zero at
+ * k_hi[i] (the last observed step, n_steps - 1).  psi_k comes from psi_hi + hi_off[i] (the
+ * state vector psi_{k_hi}) by inverse Crank-Nicolson steps, replaced by the checkpoint
+ * ckpt + ckpt_off[i] + (m - 1) * state_length whenever k = m * every (m = 1 .. n_ckpt; ckpt
+ * may be NULL).  One call covers the whole backward pass; it keeps two state vectors per
+ * resident workgroup and stores no states.  Device arrays; offsets are host arrays in doubles.
+ */
+int ctdse_adjoint_reverse_batch(int n_inst, const ctdse_geometry *const *geom, const int *k_hi, const double *field,
+                                const long long *field_off, const double *gobs, const long long *gobs_off,
+                                const double *psi_hi, const long long *hi_off, const double *ckpt,
+                                const long long *ckpt_off, int every, int n_ckpt, double *grad,
+                                const long long *grad_off, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

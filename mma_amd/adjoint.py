@@ -3,11 +3,14 @@ reference, which has no gradient code).
 
 ``TDSEDipole.apply(E, geometry, n_steps)`` returns the dipole acceleration (sourceterm,
 prop.c:328-333) of a batch of instances as a differentiable torch function of the sampled
-field E.  Its backward pass is the hand-written adjoint Crank-Nicolson kernel
-(tdse_adjoint.hip): the forward stores the wavefunction every K ~ sqrt(n_steps) steps,
-the backward recomputes each block of K states from its checkpoint and sweeps the adjoint
-through it.  The field gating of prop.c:125-134 (zero until the first sign change) is held
-fixed: gated samples get zero gradient.
+field E.  The forward stores the wavefunction every K ~ sqrt(n_steps) steps.  The backward
+pass is a hand-written adjoint Crank-Nicolson kernel, in one of two modes:
+  * "reconstruct" (default, tdse_reverse.hip): one sweep rebuilds psi_k by inverse steps,
+    re-anchored at the checkpoints, next to the adjoint state; no states are stored;
+  * "stored" (tdse_adjoint.hip): each block of K states is recomputed from its checkpoint
+    into memory and the adjoint sweeps through it.
+The field gating of prop.c:125-134 (zero until the first sign change) is held fixed: gated
+samples get zero gradient.
 
 ``harmonic_yield`` is the windowed harmonic yield sum_{w in window} |F[s](w)|^2 in the
 calcFFTW3 convention (1DTDSE/sources/tools_fftw3.c:103-131), differentiable through torch.fft.
@@ -29,6 +32,11 @@ def _bind():
         L.ctdse_propagate_states.argtypes = [c_int, POINTER(c_void_p), POINTER(c_int), POINTER(c_int), POINTER(c_int),
                                              c_void_p, POINTER(c_longlong), c_void_p, POINTER(c_longlong), c_void_p,
                                              POINTER(c_longlong), c_int, c_void_p, POINTER(c_longlong), c_void_p]
+        L.ctdse_adjoint_reverse_batch.restype = c_int
+        L.ctdse_adjoint_reverse_batch.argtypes = [c_int, POINTER(c_void_p), POINTER(c_int), c_void_p,
+                                                  POINTER(c_longlong), c_void_p, POINTER(c_longlong), c_void_p,
+                                                  POINTER(c_longlong), c_void_p, POINTER(c_longlong), c_int, c_int,
+                                                  c_void_p, POINTER(c_longlong), c_void_p]
         L.ctdse_adjoint_batch.restype = c_int
         L.ctdse_adjoint_batch.argtypes = [c_int, POINTER(c_void_p), POINTER(c_int), POINTER(c_int), c_void_p,
                                           POINTER(c_longlong), c_void_p, POINTER(c_longlong), c_void_p,
@@ -106,18 +114,39 @@ def run_adjoint(geometry, F, gobs, states, kappa, grad, k_hi, k_lo):
     tdse._raise(code)
 
 
+def run_adjoint_reverse(geometry, F, gobs, psi_hi, ckpt, every, grad, k_hi):
+    """The whole backward sweep k = k_hi .. 1 in one call: psi_hi (n_inst, S, 2), rows contiguous
+    (a slice of the checkpoints will do), is psi_{k_hi}; ckpt (n_inst, m, S, 2) holds
+    psi_{(j+1) every} in slot j, or None; grad (n_inst, >= k_hi)."""
+    L = _bind()
+    n_inst, Lf = F.shape
+    S = state_length(geometry)
+    n_ckpt = ckpt.shape[1] if ckpt is not None else 0
+    ckpt_p = c_void_p(ckpt.data_ptr()) if ckpt is not None else c_void_p()
+    code = L.ctdse_adjoint_reverse_batch(n_inst, _ptrs([geometry] * n_inst), _i([k_hi] * n_inst),
+                                         c_void_p(F.data_ptr()), _ll([i * Lf for i in range(n_inst)]),
+                                         c_void_p(gobs.data_ptr()), _ll([i * gobs.shape[1] for i in range(n_inst)]),
+                                         c_void_p(psi_hi.data_ptr()), _ll([i * psi_hi.stride(0) for i in range(n_inst)]),
+                                         ckpt_p, _ll([i * n_ckpt * 2 * S for i in range(n_inst)]), int(every),
+                                         int(n_ckpt), c_void_p(grad.data_ptr()),
+                                         _ll([i * grad.shape[1] for i in range(n_inst)]), _stream(F))
+    tdse._raise(code)
+
+
 class TDSEDipole(torch.autograd.Function):
     """sourceterm(E): (n_inst, L) float64 CUDA field samples -> (n_inst, n_steps) dipole
     acceleration, n_steps = outputs.Nt of the reference (L >= n_steps + 1)."""
 
     @staticmethod
-    def forward(ctx, E, geometry, n_steps, checkpoint_every=None):
+    def forward(ctx, E, geometry, n_steps, checkpoint_every=None, mode="reconstruct"):
         if not (E.is_cuda and E.dtype == torch.float64):
             raise TypeError("E must be a float64 CUDA tensor")
         E = E.contiguous()
         n_inst, Lf = E.shape
         if Lf < n_steps + 1:
             raise ValueError("E needs n_steps + 1 samples per row")
+        if mode not in ("reconstruct", "stored"):
+            raise ValueError("mode must be 'reconstruct' or 'stored'")
         K = int(checkpoint_every or max(1, math.ceil(math.sqrt(n_steps))))
         F, mask = gate(E)
         S = state_length(geometry)
@@ -126,7 +155,7 @@ class TDSEDipole(torch.autograd.Function):
         out = torch.empty((n_inst, 5, n_steps), dtype=torch.float64, device=E.device)
         run_states(geometry, F, n_steps, 0, 0, None, ckpt if n_ckpt > 0 else None, K, out)
         ctx.save_for_backward(F, mask, ckpt)
-        ctx.geometry, ctx.n_steps, ctx.K, ctx.n_ckpt = geometry, n_steps, K, n_ckpt
+        ctx.geometry, ctx.n_steps, ctx.K, ctx.n_ckpt, ctx.mode = geometry, n_steps, K, n_ckpt, mode
         return out[:, 0, :].clone()
 
     @staticmethod
@@ -136,8 +165,24 @@ class TDSEDipole(torch.autograd.Function):
         n_inst = F.shape[0]
         S = state_length(geometry)
         gobs = gs.contiguous().to(torch.float64)
-        kappa = torch.zeros((n_inst, S, 2), dtype=torch.float64, device=F.device)
         grad = torch.zeros((n_inst, N), dtype=torch.float64, device=F.device)
+        dE = torch.zeros_like(F)
+        if N < 2:
+            return dE, None, None, None, None
+        if ctx.mode == "reconstruct":
+            # psi_{N-1}: the last checkpoint at or below N-1, run forward to N-1 if needed
+            j = (N - 1) // K
+            if j >= 1 and j * K == N - 1:
+                psi_hi = ckpt[:, j - 1]
+            else:
+                buf = torch.zeros((n_inst, 1, S, 2), dtype=torch.float64, device=F.device)
+                n_run = N - 1 - j * K
+                run_states(geometry, F, N, j * K, n_run, None if j == 0 else ckpt, buf, n_run, init_slot=j - 1)
+                psi_hi = buf[:, 0]
+            run_adjoint_reverse(geometry, F, gobs, psi_hi, ckpt if ctx.n_ckpt > 0 else None, K, grad, N - 1)
+            dE[:, :N] = grad * mask[:, :N]
+            return dE, None, None, None, None
+        kappa = torch.zeros((n_inst, S, 2), dtype=torch.float64, device=F.device)
         block = torch.zeros((n_inst, K, S, 2), dtype=torch.float64, device=F.device)
         n_blocks = (N - 2) // K + 1  # blocks cover k = 1 .. N-1
         for m in range(n_blocks - 1, -1, -1):
@@ -147,9 +192,8 @@ class TDSEDipole(torch.autograd.Function):
                 continue
             run_states(geometry, F, N, k_lo, k_hi - k_lo, None if m == 0 else ckpt, block, 1, init_slot=m - 1)
             run_adjoint(geometry, F, gobs, block, kappa, grad, k_hi, k_lo)
-        dE = torch.zeros_like(F)
         dE[:, :N] = grad * mask[:, :N]
-        return dE, None, None, None
+        return dE, None, None, None, None
 
 
 def harmonic_window(n, dt, tmax, omega0, h_lo, h_hi):
@@ -167,4 +211,5 @@ def harmonic_yield(s, dt, window):
     return (P * window.to(P.device, P.dtype)).sum(dim=-1)
 
 
-__all__ = ["TDSEDipole", "gate", "harmonic_yield", "harmonic_window", "run_states", "run_adjoint", "state_length"]
+__all__ = ["TDSEDipole", "gate", "harmonic_yield", "harmonic_window", "run_states", "run_adjoint",
+           "run_adjoint_reverse", "state_length"]

@@ -27,6 +27,7 @@ SOURCES = [
     ("tdse_analysis.hip", ["-ffp-contract=off"]),
     ("tdse_propagate.hip", []),
     ("tdse_adjoint.hip", []),
+    ("tdse_reverse.hip", []),
     ("ctdse_abi.cpp", ["-x", "hip"]),
     ("hankel_medium.hip", []),
     ("hankel_contract.hip", []),
@@ -36,7 +37,7 @@ SOURCES = [
 
 LIBS = {
     "libsingleTDSE.so": ["tdse_setup.hip", "tdse_analysis.hip", "tdse_propagate.hip", "tdse_adjoint.hip",
-                         "ctdse_abi.cpp"],
+                         "tdse_reverse.hip", "ctdse_abi.cpp"],
     "libmmahankel.so": ["hankel_medium.hip", "hankel_contract.hip", "hankel_gemm.hip", "hankel_abi.cpp"],
 }
 

@@ -404,6 +404,60 @@ int adjoint(const std::vector<const ctdse_geometry *> &ig, const std::vector<Tds
     return CTDSE_OK;
 }
 
+int reverse_adjoint(const std::vector<const ctdse_geometry *> &ig, const std::vector<TdseRevInst> &ri,
+                    hipStream_t stream)
+{
+    const int n = (int)ri.size();
+    if (n == 0) return CTDSE_OK;
+    std::vector<const ctdse_geometry *> geoms;
+    std::vector<TdseRevInst> insts = ri;
+    long long stride = 0;
+    for (int i = 0; i < n; i++) {
+        int gi = (int)(std::find(geoms.begin(), geoms.end(), ig[i]) - geoms.begin());
+        if (gi == (int)geoms.size()) geoms.push_back(ig[i]);
+        insts[i].geom = gi;
+        stride = std::max(stride, ig[i]->dev.npad);
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+        return (double)ig[p]->dev.nx * ri[p].k_hi > (double)ig[q]->dev.nx * ri[q].k_hi;
+    });
+    std::vector<TdseGeom> gdev(geoms.size());
+    for (size_t k = 0; k < geoms.size(); k++) gdev[k] = geoms[k]->dev;
+    int min_nw = 1, max_h = 0;
+    for (const ctdse_geometry *g : geoms) max_h = std::max(max_h, g->dev.halo_lanes);
+    const int nw = pick_waves(&min_nw, max_h);
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    const int per_cu = std::max(1, 4 * TDSE_REV_WAVES_PER_SIMD / nw);
+    const int slots = std::max(1, std::min(n, prop.multiProcessorCount * per_cu));
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    const size_t need = (size_t)slots * 2 * stride * sizeof(double2);  // psi and kappa per workgroup
+    if (need > g_work_bytes) {
+        HIP_CHECK(g_work.alloc(need));
+        g_work_bytes = need;
+    }
+    DevMem d_geom, d_inst, d_order, d_counter;
+    HIP_CHECK(d_geom.alloc(gdev.size() * sizeof(TdseGeom)));
+    HIP_CHECK(d_inst.alloc(insts.size() * sizeof(TdseRevInst)));
+    HIP_CHECK(d_order.alloc(order.size() * sizeof(int)));
+    HIP_CHECK(d_counter.alloc(sizeof(int)));
+    HIP_CHECK(hipMemcpyAsync(d_geom.p, gdev.data(), gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_inst.p, insts.data(), insts.size() * sizeof(TdseRevInst), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, d_counter.as<int>(), 1, 0);
+    const size_t park = (size_t)nw * TDSE_TILE * sizeof(double2);  // one parked state chunk per lane
+    hipLaunchKernelGGL(tdse_reverse_adjoint_kernel, dim3(slots), dim3(64 * nw), park, stream, d_geom.as<TdseGeom>(),
+                       d_inst.as<TdseRevInst>(), d_order.as<int>(), n, d_counter.as<int>(), g_work.as<double2>(),
+                       stride);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return CTDSE_OK;
+}
+
 double *host_copy(const double *dev, size_t n, size_t alloc_n)
 {
     double *h = (double *)calloc(std::max<size_t>(alloc_n, 1), sizeof(double));
@@ -553,6 +607,37 @@ int ctdse_adjoint_batch(int n_inst, const ctdse_geometry *const *geom, const int
         a.grad = grad + grad_off[i];
     }
     return adjoint(ig, ai, (hipStream_t)stream);
+}
+
+int ctdse_adjoint_reverse_batch(int n_inst, const ctdse_geometry *const *geom, const int *k_hi, const double *field,
+                                const long long *field_off, const double *gobs, const long long *gobs_off,
+                                const double *psi_hi, const long long *hi_off, const double *ckpt,
+                                const long long *ckpt_off, int every, int n_ckpt, double *grad,
+                                const long long *grad_off, void *stream)
+{
+    clear_status();
+    if (n_inst < 0 || (n_inst > 0 && (!geom || !k_hi || !field || !field_off || !gobs || !gobs_off || !psi_hi ||
+                                      !hi_off || !grad || !grad_off || (ckpt && !ckpt_off))))
+        return set_status(CTDSE_ERR_ARGUMENT, "null adjoint argument");
+    if (ckpt && (every < 1 || n_ckpt < 0)) return set_status(CTDSE_ERR_ARGUMENT, "bad checkpoint spacing");
+    if (device_ready() != CTDSE_OK) return g_status;
+    std::vector<const ctdse_geometry *> ig(n_inst);
+    std::vector<TdseRevInst> ri(n_inst);
+    for (int i = 0; i < n_inst; i++) {
+        if (!geom[i] || k_hi[i] < 1)
+            return set_status(CTDSE_ERR_ARGUMENT, "adjoint instance " + std::to_string(i) + ": bad range");
+        ig[i] = geom[i];
+        TdseRevInst &r = ri[i];
+        r.k_hi = k_hi[i];
+        r.field = field + field_off[i];
+        r.gobs = gobs + gobs_off[i];
+        r.psi_hi = (const double2 *)(psi_hi + hi_off[i]);
+        r.ckpt = ckpt ? (const double2 *)(ckpt + ckpt_off[i]) : nullptr;
+        r.every = ckpt ? every : 1;
+        r.n_ckpt = ckpt ? n_ckpt : 0;
+        r.grad = grad + grad_off[i];
+    }
+    return reverse_adjoint(ig, ri, (hipStream_t)stream);
 }
 
 void Initialise_grid_and_ground_state(inputs_def *in)

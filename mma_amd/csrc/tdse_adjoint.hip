@@ -24,7 +24,7 @@ using namespace tdse;
 
 // edge_in / edge_out: {old kappa, y} at the last own point of the previous / this segment.
 __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
-                                double2 *__restrict__ kap, const double2 *__restrict__ psik, double gobs, double Fdt,
+                                double2 *__restrict__ kap, const double2 *__restrict__ psik, double gobs, double2 e0,
                                 double2 rot_conj, double &gacc, double2 (*slot_f)[2], double2 (*slot_b)[2],
                                 const double2 *edge_in, double2 *edge_out)
 {
@@ -68,10 +68,7 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
     // lambda = A'^H kappa + source, the gradient term, then the conjugate phase
     {
         const int j0 = c * C;
-        const double x0 = __dsub_rn(__dmul_rn((double)j0, G.dx), G.xmax);
-        double s0, c0;
-        sincos(-Fdt * x0, &s0, &c0);
-        double2 e = make_double2(c0, s0);
+        double2 e = e0;  // exp(-i F dt x) at the chunk's first point
         const double K = G.k36;
         double2 pm = left;
         double gsum = 0.0;
@@ -156,7 +153,8 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
 
 // One workgroup per instance at a time (atomic queue), all waves on one segment.
 __global__ void __launch_bounds__(64 * TDSE_MAX_WAVES)
-tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *insts, const int *order, int n_inst, int *counter)
+tdse_adjoint_kernel(const TdseGeom *__restrict__ geoms, const TdseAdjInst *__restrict__ insts,
+                    const int *__restrict__ order, int n_inst, int *counter)
 {
     __shared__ double part[2][TDSE_MAX_WAVES];
     __shared__ double2 slot_f[TDSE_MAX_WAVES][2], slot_b[TDSE_MAX_WAVES][2];
@@ -182,12 +180,23 @@ tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *insts, const int *
             double rs, rc;
             sincos(-Fdt * G.dx, &rs, &rc);
             const double2 rot_conj = make_double2(rc, rs);
+            // the phase at the first point of this lane's chunk in segment 0, and its rotation
+            // from one segment to the next
+            double2 e_lane, r_seg;
+            {
+                const double x = __dsub_rn(__dmul_rn((double)((wave * 64 + lane) * C), G.dx), G.xmax);
+                sincos(-Fdt * x, &e_lane.y, &e_lane.x);
+                sincos(-Fdt * ((double)(own_chunks * C) * G.dx), &r_seg.y, &r_seg.x);
+            }
             const double gobs = I.gobs[k];
             const double2 *psik = I.psi + (long long)(k - I.k_lo - 1) * npad;
             double gacc = 0.0;
-            for (int s = 0; s < nseg; s++)
-                adjoint_segment(G, s, own_chunks, wave, lane, nw, I.kappa, psik, gobs, Fdt, rot_conj, gacc, slot_f,
+            double2 e0 = e_lane;
+            for (int s = 0; s < nseg; s++) {
+                adjoint_segment(G, s, own_chunks, wave, lane, nw, I.kappa, psik, gobs, e0, rot_conj, gacc, slot_f,
                                 slot_b, edge[s & 1], edge[(s + 1) & 1]);
+                e0 = cmul(e0, r_seg);
+            }
             const double gw = wave_sum(gacc);
             if (lane == 0) part[k & 1][wave] = gw;
             __syncthreads();

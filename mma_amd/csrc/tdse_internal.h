@@ -21,6 +21,7 @@
 #define TDSE_WAVES_PER_SIMD 3      // propagation kernel: register budget 168 VGPRs, 12 waves per CU
 #define TDSE_WAVES_PER_CU (4 * TDSE_WAVES_PER_SIMD)
 #define TDSE_ADJ_WAVES_PER_CU 8    // adjoint kernel: 2 waves per SIMD (224 VGPRs)
+#define TDSE_REV_WAVES_PER_SIMD 2  // reconstructing adjoint kernel: 256-VGPR budget, 8 waves per CU
 #define TDSE_MAX_HALO 40           // largest truncation depth in chunks
 
 // storage offset of chunk c (c >= -64), row i of the chunk
@@ -90,6 +91,20 @@ struct TdseAdjInst {
     double *grad;             // dL/dF_k, written for k = k_hi-1 down to k_lo
 };
 
+// One instance of a reconstructing adjoint sweep (tdse_reverse_adjoint_kernel): psi_k is
+// rebuilt backward from psi_{k_hi} by inverse Crank-Nicolson steps, re-anchored at the
+// checkpoints, while kappa_k runs backward from kappa_{k_hi} = 0.
+struct TdseRevInst {
+    int geom;
+    int k_hi;                 // sweeps k = k_hi down to 1 (k_hi <= n_steps - 1 of the forward)
+    const double *field;      // gated field F_k
+    const double *gobs;       // dL/d sourceterm_k, k <= k_hi
+    const double2 *psi_hi;    // tiled psi_{k_hi}
+    const double2 *ckpt;      // tiled psi_{(m+1) every} at ckpt + m * npad, m < n_ckpt (may be null)
+    int every, n_ckpt;
+    double *grad;             // dL/dF_k, written for k = k_hi - 1 down to 0
+};
+
 // Ground-state job: inverse iteration in one lane (bit-identical to the CPU reference).
 struct GsJob {
     int num_r, max_iter;
@@ -124,6 +139,8 @@ __global__ void tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *ins
                                       int *counter, double2 *work, long long work_stride);
 __global__ void tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *insts, const int *order, int n_inst,
                                     int *counter);
+__global__ void tdse_reverse_adjoint_kernel(const TdseGeom *geoms, const TdseRevInst *insts, const int *order,
+                                            int n_inst, int *counter, double2 *work, long long work_stride);
 __global__ void tdse_fill_int_kernel(int *p, int n, int v);
 __global__ void tdse_xgrid_kernel(double *x, int num_r, double dx);
 __global__ void tdse_tgrid_kernel(double *tgrid, int n, double tmin, double dt);

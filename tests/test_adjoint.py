@@ -4,7 +4,7 @@ CPU: the oracle gradient (oracle/adjoint.py, banded LAPACK solves on the referen
 matrices) is pinned by central finite differences (h = 1e-6) of the C oracle's dipole, for a
 random weighting of the dipole samples and for the windowed harmonic yield; the torch gate and
 harmonic yield agree with their numpy restatements.
-GPU: TDSEDipole (forward kernel + adjoint kernel through the C ABI) against the oracle
+GPU: TDSEDipole (forward kernel + adjoint kernels through the C ABI) against the oracle
 gradient, against finite differences of its own forward, and independent of the checkpoint
 interval."""
 import math
@@ -166,17 +166,27 @@ def test_gpu_gradient_finite_differences(gpu_case):
 
 @pytest.mark.gpu
 def test_gpu_gradient_independent_of_checkpoint_interval(gpu_case):
+    """Stored states: the same arithmetic for every checkpoint interval K (bitwise).  Rebuilt
+    states (inverse steps from the nearest later checkpoint, up to K = N steps back): equal to
+    the stored-state gradient to rounding."""
     from mma_amd import adjoint as AD
     geo, x, psi0, fields = gpu_case
     gobs = torch.tensor(np.random.default_rng(9).standard_normal((len(fields), GPU_N)), device="cuda")
-    grads = []
-    for K in (1, 7, 20, GPU_N - 1, GPU_N):
+
+    def grad_of(K, mode):
         E = torch.tensor(fields, device="cuda", requires_grad=True)
-        (AD.TDSEDipole.apply(E, geo, GPU_N, K) * gobs).sum().backward()
-        grads.append(E.grad.cpu().numpy())
-        print("K", K, flush=True)
-    for g in grads[1:]:
-        assert np.array_equal(g, grads[0])
+        (AD.TDSEDipole.apply(E, geo, GPU_N, K, mode) * gobs).sum().backward()
+        return E.grad.cpu().numpy()
+
+    Ks = (1, 7, 20, GPU_N - 1, GPU_N)
+    stored = [grad_of(K, "stored") for K in Ks]
+    for g in stored[1:]:
+        assert np.array_equal(g, stored[0])
+    for K in Ks:
+        g = grad_of(K, "reconstruct")
+        e = rel_err(g, stored[0])
+        print("K %d rebuilt vs stored %.2e" % (K, e), flush=True)
+        assert e < 1e-11
 
 
 # ------------------------------------------------- gradient workload (Nx = 8001, dt = 0.1)
