@@ -344,15 +344,8 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
     HIP_CHECK(hipMemcpyAsync(d_inst.p, insts.data(), insts.size() * sizeof(TdseInst), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(d_order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, d_counter.as<int>(), 1, 0);
-    bool any_save = false, any_snap = false;
-    for (const InstSpec &s : spec) {
-        any_save = any_save || s.save;
-        any_snap = any_snap || (s.snap && s.n_snap > 0);
-    }
-    auto kern = any_save ? (any_snap ? tdse_propagate_kernel<true, true> : tdse_propagate_kernel<true, false>)
-                         : (any_snap ? tdse_propagate_kernel<false, true> : tdse_propagate_kernel<false, false>);
-    hipLaunchKernelGGL(kern, dim3(slots), dim3(64 * nw), 0, stream, d_geom.as<TdseGeom>(), d_inst.as<TdseInst>(),
-                       d_order.as<int>(), n, d_counter.as<int>(), g_work.as<double2>(), stride);
+    hipLaunchKernelGGL(tdse_propagate_kernel, dim3(slots), dim3(64 * nw), 0, stream, d_geom.as<TdseGeom>(),
+                       d_inst.as<TdseInst>(), d_order.as<int>(), n, d_counter.as<int>(), g_work.as<double2>(), stride);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(stream));
     return CTDSE_OK;

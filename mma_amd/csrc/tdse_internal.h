@@ -134,7 +134,6 @@ __global__ void tdse_ground_state_kernel(GsJob *jobs, int n_jobs);
 __global__ void tdse_tables_grid_kernel(TableJob *job);
 __global__ void tdse_tables_pivot_kernel(TableJob *job);
 __global__ void tdse_tables_chunk_kernel(TableJob *job);
-template <bool SAVE, bool SNAP>
 __global__ void tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
                                       int *counter, double2 *work, long long work_stride);
 __global__ void tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *insts, const int *order, int n_inst,

@@ -97,16 +97,13 @@ __device__ inline void load_tables(const TdseGeom &G, int seg, int own_chunks, i
 // Solve one segment of one step with all waves of the workgroup.
 // observe: add the observables of the state the step starts from (psi_k, read for the
 // solve anyway) to acc; the step's own result is observed by the next step.
-// snap_row != nullptr: also store the new psi there (natural order).
 // Each lane keeps its chunk of psi and the chunk's coefficients (g_j, 1/beta_j) in
 // registers for the whole solve: all loads of a segment are issued together, once.
 // edge_in / edge_out: {old psi, forward z} at the last own point of the previous / this
 // segment (LDS, alternating between segments).
-template <bool SAVE, bool SNAP>
 __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
                               double2 *__restrict__ psi, double2 e0, double2 rot, bool observe, Acc &acc,
-                              double *snap_row, double2 (*slot_f)[2], double2 (*slot_b)[2], const double2 *edge_in,
-                              double2 *edge_out, double2 *__restrict__ save_to)
+                              double2 (*slot_f)[2], double2 (*slot_b)[2], const double2 *edge_in, double2 *edge_out)
 {
     const int gl = wave * 64 + lane;                 // lane index in the workgroup
     const int cs = seg * own_chunks;                 // first own chunk
@@ -228,22 +225,8 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     double2 e = e0;
 #pragma unroll
     for (int i = 0; i < C; i++) {
-        const double2 v = cmul(e, P[i]);
-        P[i] = v;
-        st2(psi, base + 64 * i, v);
-        if constexpr (SAVE) {
-            if (save_to) st2(save_to, base + 64 * i, v);
-        }
+        st2(psi, base + 64 * i, cmul(e, P[i]));
         e = cmul(e, rot);
-    }
-    if (SNAP && snap_row) {
-#pragma unroll
-        for (int i = 0; i < C; i++) {
-            if (j0 + i < G.nx) {
-                snap_row[2 * (j0 + i)] = P[i].x;
-                snap_row[2 * (j0 + i) + 1] = P[i].y;
-            }
-        }
     }
 }
 
@@ -303,9 +286,8 @@ __device__ inline void store_observables(const TdseInst &I, const double (*part)
 
 } // namespace
 
-// SAVE: some instance stores states (partial runs); SNAP: some instance takes snapshots.
-// Both are off for plain batches, which then carry neither store in the step loop.
-template <bool SAVE, bool SNAP>
+// States saved for partial runs and snapshots are copied from the finished state between
+// steps (they are rare), so the segment solve carries no extra stores.
 __global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_WAVES_PER_SIMD)
 tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
                       const int *__restrict__ order, int n_inst, int *counter, double2 *work, long long work_stride)
@@ -358,7 +340,7 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
                 F = fk;
             }
             double2 *save_to = nullptr;
-            if (SAVE && I.save && (k - k_start + 1) % I.save_every == 0)
+            if (I.save && (k - k_start + 1) % I.save_every == 0)
                 save_to = I.save + (long long)((k - k_start + 1) / I.save_every - 1) * npad;
             const double Fdt = F * G.dt;
             double rs, rc;
@@ -374,7 +356,7 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
             }
             // snapshots, prop.c:248-255: row i_snap gets psi after step k when k % every == 0
             double *snap_row = nullptr;
-            if (SNAP && I.snap && k % I.snap_every == 0 && i_snap < I.n_snap - 1) {
+            if (I.snap && k % I.snap_every == 0 && i_snap < I.n_snap - 1) {
                 snap_row = I.snap + (long long)i_snap * 2 * G.nx;
                 i_snap++;
             }
@@ -384,14 +366,25 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
             Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
             double2 e0 = e_lane;
             for (int s = 0; s < nseg; s++) {
-                solve_segment<SAVE, SNAP>(G, s, own_chunks, wave, lane, nw, buf, e0, rot, observe, acc, snap_row,
-                                          slot_f, slot_b, edge[s & 1], edge[(s + 1) & 1], save_to);
+                solve_segment(G, s, own_chunks, wave, lane, nw, buf, e0, rot, observe, acc, slot_f, slot_b,
+                              edge[s & 1], edge[(s + 1) & 1]);
                 e0 = cmul(e0, r_seg);
             }
             wave_observables(acc, wave, lane, part[k & 1]);
             __syncthreads();  // also: psi of step k + 1 is complete in memory
             if (threadIdx.x == 0 && I.src && observe) store_observables(I, part[k & 1], nw, k, F_prev);
             F_prev = F;
+            if (save_to || snap_row) {
+                if (save_to)
+                    for (long long q = threadIdx.x; q < npad; q += blockDim.x) save_to[q] = buf[q];
+                if (snap_row)
+                    for (int j = threadIdx.x; j < G.nx; j += blockDim.x) {
+                        const double2 v = buf[tdse_index(j)];
+                        snap_row[2 * j] = v.x;
+                        snap_row[2 * j + 1] = v.y;
+                    }
+                __syncthreads();  // the copies have read psi before the next step overwrites it
+            }
         }
         // the final state of a partial run: entry k_stop (a full run has no entry n_steps)
         if (k_stop > k_start && k_stop < I.n_steps) {
@@ -422,12 +415,3 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
         __syncthreads();
     }
 }
-
-template __global__ void tdse_propagate_kernel<false, false>(const TdseGeom *, const TdseInst *, const int *, int, int *,
-                                                             double2 *, long long);
-template __global__ void tdse_propagate_kernel<true, false>(const TdseGeom *, const TdseInst *, const int *, int, int *,
-                                                            double2 *, long long);
-template __global__ void tdse_propagate_kernel<false, true>(const TdseGeom *, const TdseInst *, const int *, int, int *,
-                                                            double2 *, long long);
-template __global__ void tdse_propagate_kernel<true, true>(const TdseGeom *, const TdseInst *, const int *, int, int *,
-                                                           double2 *, long long);
