@@ -76,3 +76,23 @@ def test_hankel_ctypes_structs_match_header():
         assert ctypes.sizeof(cls) == got[name]["size"], name
         for f, off in got[name]["fields"].items():
             assert getattr(cls, f).offset == off, (name, f)
+
+
+def test_batched_entry_points_reject_bad_arguments_before_touching_a_device():
+    """Argument errors of the batched and gradient entry points come back as status 3
+    (CTDSE_ERR_ARGUMENT) with a message; the checks run before any device call."""
+    from mma_amd import library_path
+    lib = ctypes.CDLL(library_path())
+    lib.ctdse_last_error_message.restype = ctypes.c_char_p
+    N = None
+    assert lib.ctdse_adjoint_reverse_batch(1, N, N, N, N, N, N, N, N, N, N, 1, 0, N, N, N) == 3
+    assert b"null" in lib.ctdse_last_error_message()
+    assert lib.ctdse_last_error() == 3
+    dummy = (ctypes.c_longlong * 1)(0)
+    p = ctypes.cast(dummy, ctypes.c_void_p)
+    assert lib.ctdse_adjoint_reverse_batch(1, p, p, p, p, p, p, p, p, p, p, 0, 1, p, p, N) == 3
+    assert b"checkpoint spacing" in lib.ctdse_last_error_message()
+    assert lib.ctdse_adjoint_batch(1, N, N, N, N, N, N, N, N, N, N, N, N, N, N) == 3
+    assert lib.ctdse_propagate_states(1, N, N, N, N, N, N, N, N, N, N, 1, N, N, N) == 3
+    assert lib.ctdse_propagate_batch(1, N, N, N, N, N, N, N) == 3
+    assert lib.ctdse_adjoint_reverse_batch(0, N, N, N, N, N, N, N, N, N, N, 1, 0, N, N, N) in (0, 5)  # empty batch
