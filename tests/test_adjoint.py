@@ -178,7 +178,7 @@ def test_gpu_gradient_independent_of_checkpoint_interval(gpu_case):
         (AD.TDSEDipole.apply(E, geo, GPU_N, K, mode) * gobs).sum().backward()
         return E.grad.cpu().numpy()
 
-    Ks = (1, 7, 20, GPU_N - 1, GPU_N)
+    Ks = (1, 7, 20, GPU_N - 1, GPU_N, GPU_N + 3)  # K > N: no checkpoint at all
     stored = [grad_of(K, "stored") for K in Ks]
     for g in stored[1:]:
         assert np.array_equal(g, stored[0])
