@@ -241,6 +241,83 @@ def hankel_measure(cpu=True):
     return info
 
 
+def uneven_measure(steps=2):
+    """Uneven workload: 8192 instances drawn from a seeded PCG64 generator, each with its own
+    grid (Nx in [4001, 32001]), duration (6-20 cycles), peak intensity (5e13-3e14 W/cm^2,
+    log-uniform) and wavelength (800-1800 nm), dt = 0.25.  All ground states and tables are
+    built in grouped launches (timed as setup), the launch shape comes from the autotuner,
+    and the batch runs as one persistent launch replayed from a HIP graph."""
+    import torch
+    from mma_amd import autotune, tdse, workloads as W
+    U = W.UNEVEN
+    members = W.uneven_batch(**U)
+    fields, n_steps = W.stack_fields([m["field"] for m in members])
+    num_r = [m["num_r"] for m in members]
+    del members
+    t0 = time.perf_counter()
+    geoms = tdse.Geometry.create_many(num_r, 0.1, U["dt"], W.GRID["a"], W.GRID["e_guess"], W.GRID["cv"],
+                                      W.GRID["x_int"])
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    log("uneven: %d geometries built in %.2f s" % (len(geoms), t_setup))
+    dev = torch.tensor(fields, device="cuda")
+    del fields
+    out = torch.zeros((len(geoms), 5, max(n_steps)), dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    res = autotune.tune(geoms, dev, n_steps=n_steps)
+    t_tune = time.perf_counter() - t0
+    log("uneven: tuned %r in %.2f s (cached %s)" % (res.config, t_tune, res.cached))
+    plan = tdse.BatchPlan(geoms, dev, n_steps=n_steps, out=out, config=res.config)
+    default = tdse.default_config(geoms)
+    stream = torch.cuda.current_stream()
+    plan.replay(stream)  # captures the graph; warm-up
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        plan.replay(stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    kern = float(np.mean([a.elapsed_time(b) for a, b in ev])) * 1e-3
+    ok = bool(torch.isfinite(out).all()) and float(out[:, 1, :].max()) <= 1.0 + 1e-12
+    pt = float(plan.pt_steps)
+    log("uneven: %.3f s per batch, %.3g pt-steps/s" % (wall, pt / wall))
+    work = np.array([(r + 1) * s for r, s in zip(num_r, n_steps)], dtype=float)
+    info = {"workload": "tdse-batch-uneven", "instances": len(geoms), "seed": U["seed"],
+            "Nx_range": [int(min(num_r)) + 1, int(max(num_r)) + 1], "distinct_Nx": len(set(num_r)),
+            "steps_range": [int(min(n_steps)), int(max(n_steps))], "pt_steps": pt,
+            "max_over_min_instance_work": float(work.max() / work.min()),
+            "value": pt / wall, "unit": "grid-pt-steps/s", "seconds": wall, "kernel_s": kern,
+            "hbm_frac_psi_only": BYTES_PER_PT_STEP * pt / kern / 1e9 / HBM_PEAK_GBS,
+            "setup_s": t_setup, "tune_s": t_tune, "launch_config": list(res.config.as_tuple()),
+            "default_config": list(default.as_tuple()), "tune_table": [list(r) for r in res.table[:6]],
+            "workgroups": plan.workgroups, "launch": "hipGraph replay of one persistent launch",
+            "outputs_finite_and_norm_conserved": ok}
+    plan.close()
+    del out, dev, geoms
+    return info
+
+
+def stepwise_measure(geo, dev, n_steps, ref):
+    """The plain form of the headline batch: one kernel launch per time step over all
+    instances, states in HBM between launches (the same segment solve, in the default launch
+    shape).  Returns the wall time and max |dipole difference| / max |dipole| against ref."""
+    import torch
+    from mma_amd import tdse
+    out = torch.empty((dev.shape[0], 5, n_steps), dtype=torch.float64, device="cuda")
+    tdse.propagate_stepwise(geo, dev[:, :3].contiguous(), out=out[:, :, :2].contiguous())  # load paths
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tdse.propagate_stepwise(geo, dev, out=out)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    diff = float((out[:, 0] - ref[:, 0]).abs().max() / ref[:, 0].abs().max())
+    del out
+    return wall, diff
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +328,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the 8192-instance uneven workload")
+    ap.add_argument("--no-stepwise", action="store_true", help="skip the one-launch-per-step form")
+    ap.add_argument("--only-uneven", action="store_true", help="time only the uneven workload")
     ap.add_argument("--adj-inst", type=int, default=1024)
     ap.add_argument("--adj-samples", type=int, default=40000)
     ap.add_argument("--only-adjoint", action="store_true", help="time only the gradient workload")
@@ -271,6 +351,9 @@ def main():
     if args.only_hankel:
         print(json.dumps({"hankel": hankel_measure(cpu=not args.no_cpu)}), flush=True)
         return
+    if args.only_uneven:
+        print(json.dumps({"uneven": uneven_measure()}), flush=True)
+        return
     if args.only_adjoint:
         import torch
         print(json.dumps({"adjoint": adjoint_measure(args.adj_inst, args.adj_samples)}), flush=True)
@@ -281,7 +364,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from mma_amd import tdse, workloads as W
+    from mma_amd import autotune, tdse, workloads as W
 
     g = W.GRID
     log("setup: ground state + factor tables")
@@ -293,19 +376,26 @@ def main():
     dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
     out = torch.empty((n_inst, 5, n_steps), dtype=torch.float64, device="cuda")
     pt_steps = float(n_inst) * nx * n_steps
+    # launch shape from the autotuner (a few steps of every instance per candidate), then one
+    # persistent launch per bench step, replayed from a HIP graph
+    t_tune = time.perf_counter()
+    tuned = autotune.tune(geo, dev, n_steps=[n_steps] * n_inst)
+    t_tune = time.perf_counter() - t_tune
+    log("launch shape %r (cached %s, %.2f s)" % (tuned.config, tuned.cached, t_tune))
+    plan = tdse.BatchPlan(geo, dev, n_steps=[n_steps] * n_inst, out=out, config=tuned.config)
+    stream = torch.cuda.current_stream()
     for w in range(args.warmup):
         log("warmup %d" % w)
-        tdse.propagate_batch(geo, dev, out=out)
+        plan.replay(stream)
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     t0 = time.perf_counter()
     for k in range(args.steps):
         ev[k][0].record(stream)
-        tdse.propagate_batch(geo, dev, out=out)
+        plan.replay(stream)
         ev[k][1].record(stream)
-        log("step %d done" % k)
     torch.cuda.synchronize()
+    log("%d steps done" % args.steps)
     wall = time.perf_counter() - t0
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     ms_per_step = wall / args.steps * 1e3
@@ -313,6 +403,15 @@ def main():
     achieved = BYTES_PER_PT_STEP * pt_steps / (kern_ms * 1e-3) / 1e9
     src = out[:, 0, :].cpu().numpy()
     finite = bool(np.all(np.isfinite(src)))
+
+    stepwise = None
+    if not args.no_stepwise:
+        log("stepwise form")
+        sw_wall, sw_diff = stepwise_measure(geo, dev, n_steps, out)
+        stepwise = {"pt_steps_per_s": pt_steps / sw_wall, "seconds": sw_wall, "launches": n_steps,
+                    "dipole_max_rel_diff_vs_persistent": sw_diff}
+        log("stepwise: %.3f s" % sw_wall)
+    plan.close()
 
     single = None
     if not args.no_single:
@@ -325,6 +424,13 @@ def main():
         tdse.propagate_batch(geo, one, out=o1)
         torch.cuda.synchronize()
         single = nx * n_steps / (time.perf_counter() - s0)
+
+    uneven = None
+    if not args.no_uneven:
+        out = None
+        torch.cuda.empty_cache()
+        uneven = uneven_measure()
+        torch.cuda.empty_cache()
 
     adj = None
     if not args.no_adjoint:
@@ -354,6 +460,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC_BYTES if (args.n_r, args.n_z) == (64, 64) else None},
         "cpu_baseline": cpu,
+        "launch": {"form": "hipGraph replay of one persistent launch", "config": list(tuned.config.as_tuple()),
+                   "tune_table": [list(r) for r in tuned.table[:6]], "tune_s": t_tune, "cached": tuned.cached},
+        "stepwise_form": stepwise,
+        "persistent_over_stepwise": (value / stepwise["pt_steps_per_s"]) if stepwise else None,
+        "uneven": uneven,
         "per_instance_pt_steps_per_s": single,
         "batched_over_per_instance": (value / single) if single else None,
         "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,

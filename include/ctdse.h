@@ -197,6 +197,57 @@ void ctdse_geometry_destroy(ctdse_geometry *g);
 int ctdse_propagate_batch(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const double *field,
                           const long long *field_off, double *out, const long long *out_off, void *stream);
 
+/*
+ * Many geometries at once (no counterpart in the reference; the task farm of
+ * MPI-TDSE.c:145 computes one ground state for all its instances): geometry i has
+ * (num_r[i], dx[i], dt[i], a[i], e_guess[i], cv[i], x_int[i]) and its ground state by
+ * inverse iteration, as ctdse_geometry_create with psi0 = NULL.  The ground states run
+ * side by side in one launch per group, then the tables.  out[i] receives the
+ * geometry (NULL on failure), status[i] its ctdse_status (CTDSE_ERR_GS_NOT_CONVERGED
+ * still yields a geometry).  Returns the first non-OK status, or CTDSE_OK.
+ */
+int ctdse_geometry_create_batch(int n, const int *num_r, const double *dx, const double *dt, const double *a,
+                                const double *e_guess, const double *cv, const double *x_int, ctdse_geometry **out,
+                                int *status);
+
+/* Launch shape of the persistent propagation kernel: waves per workgroup (one segment
+ * of the parallel tridiagonal solve covers 64 * waves_per_group chunks of 8 points) and
+ * resident workgroups per compute unit. */
+typedef struct ctdse_launch_config {
+    int waves_per_group;
+    int groups_per_cu;
+} ctdse_launch_config;
+
+/* The shape ctdse_propagate_batch uses for these geometries. */
+int ctdse_launch_config_default(int n_inst, const ctdse_geometry *const *geom, ctdse_launch_config *cfg);
+
+/*
+ * Prepared batch (no counterpart in the reference).  ctdse_batch_plan_create takes the
+ * arguments of ctdse_propagate_batch, the distance in doubles between an instance's five
+ * output rows (out_row_stride >= n_steps[i]; 0: n_steps[i], rows packed as in
+ * ctdse_propagate_batch) and a launch shape (NULL: the default); it validates them,
+ * uploads the instance descriptors and allocates a private work buffer, once.
+ *   ctdse_batch_plan_enqueue:      queues the batch on `stream` and returns at once; it
+ *                                  allocates, copies and waits for nothing, so a caller
+ *                                  may capture it into a graph.
+ *   ctdse_batch_plan_graph_launch: the first call captures the same work into a HIP graph
+ *                                  on a stream of the library's own; every call launches
+ *                                  that graph on `stream` and returns at once.
+ * Every run reads the fields and writes the outputs at the device addresses given at
+ * creation, so a caller may refill the field buffer between runs.  Runs of one plan must
+ * not overlap (they share the queue counter and the work buffer): order them on one
+ * stream.  The geometries must outlive the plan.
+ */
+typedef struct ctdse_batch_plan ctdse_batch_plan;
+int ctdse_batch_plan_create(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const double *field,
+                            const long long *field_off, double *out, const long long *out_off,
+                            long long out_row_stride, const ctdse_launch_config *config, ctdse_batch_plan **plan);
+int ctdse_batch_plan_enqueue(ctdse_batch_plan *plan, void *stream);
+int ctdse_batch_plan_graph_launch(ctdse_batch_plan *plan, void *stream);
+/* Shape in use and the number of persistent workgroups of one run. */
+int ctdse_batch_plan_info(const ctdse_batch_plan *plan, ctdse_launch_config *config, int *workgroups);
+void ctdse_batch_plan_destroy(ctdse_batch_plan *plan);
+
 /* Length, in complex doubles, of one device state vector of this geometry (the tiled
  * layout with its zero padding); states below are arrays of such vectors. */
 long long ctdse_state_length(const ctdse_geometry *g);
@@ -208,8 +259,11 @@ long long ctdse_state_length(const ctdse_geometry *g);
  * step), starting from psi_init + init_off[i] (a state vector; psi_init NULL: the ground
  * state, then k0 must be 0).  After every save_every-th step the state is written to
  * save + save_off[i] + m * state_length (m = 0, 1, ...; save may be NULL; the buffer must
- * start zeroed: only grid rows are written).  out (may be NULL) receives the observables as
- * in ctdse_propagate_batch.  All arrays are device memory; offsets are host arrays in doubles.
+ * start zeroed: only grid rows are written).  When an instance's save slot is its start
+ * state (same address) and n_run == save_every, that state vector is stepped in place
+ * instead (no copy).  out (may be NULL) receives the observables as in
+ * ctdse_propagate_batch, entries k0+1 .. k0+n_run of a partial run (entry 0 too when k0 is
+ * 0).  All arrays are device memory; offsets are host arrays in doubles.
  */
 int ctdse_propagate_states(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const int *k0,
                            const int *n_run, const double *field, const long long *field_off, const double *psi_init,

@@ -88,53 +88,79 @@ struct ctdse_geometry {
 
 namespace {
 
+constexpr size_t kSetupScratchBytes = size_t(8) << 30;  // device scratch per group of setup jobs
+constexpr int kSetupGroupJobs = 16384;                      // and at most this many jobs per group
+
+// Runs ground-state jobs (J.x and J.psi0 set) in groups of similar size, largest first: a
+// group's launch lasts as long as its largest serial inverse iteration.
+int run_gs_jobs(std::vector<GsJob> &jobs)
+{
+    std::vector<size_t> idx(jobs.size());
+    for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t p, size_t q) { return jobs[p].num_r > jobs[q].num_r; });
+    size_t i0 = 0;
+    while (i0 < idx.size()) {
+        size_t i1 = i0, bytes = 0;
+        while (i1 < idx.size() && (i1 - i0) < (size_t)kSetupGroupJobs) {
+            const size_t b = 16 * ((size_t)jobs[idx[i1]].num_r + 2) * sizeof(double);
+            if (i1 > i0 && bytes + b > kSetupScratchBytes) break;
+            bytes += b;
+            i1++;
+        }
+        const int m = (int)(i1 - i0);
+        DevMem work, d_jobs;
+        HIP_CHECK(work.alloc(bytes));
+        HIP_CHECK(d_jobs.alloc(m * sizeof(GsJob)));
+        std::vector<GsJob> group(m);
+        size_t off = 0;
+        for (int k = 0; k < m; k++) {
+            group[k] = jobs[idx[i0 + k]];
+            group[k].max_iter = kGsMaxIter;
+            group[k].work = work.as<double>() + off;
+            off += 16 * ((size_t)group[k].num_r + 2);
+        }
+        HIP_CHECK(hipMemcpy(d_jobs.p, group.data(), m * sizeof(GsJob), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(tdse_ground_state_kernel, dim3(m), dim3(64), 0, 0, d_jobs.as<GsJob>(), m);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpy(group.data(), d_jobs.p, m * sizeof(GsJob), hipMemcpyDeviceToHost));
+        for (int k = 0; k < m; k++) jobs[idx[i0 + k]] = group[k];
+        i0 = i1;
+    }
+    return CTDSE_OK;
+}
+
+int gs_status(const GsJob &J, const std::string &who)
+{
+    if (J.status == 1)
+        return set_status(CTDSE_ERR_GS_NOT_CONVERGED, who + "ground state: |dE| > CV after 1000 inverse iterations");
+    if (J.status == 2) return set_status(CTDSE_ERR_ZERO_PIVOT, who + "ground state: zero pivot in the Thomas solve");
+    return CTDSE_OK;
+}
+
 int run_ground_state(int num_r, double dx, double a, double e_guess, double cv, DevMem &x, DevMem &psi0, double *e_out,
                      int *iters)
 {
     const long long nx = num_r + 1;
-    DevMem work, job;
     HIP_CHECK(x.alloc(nx * sizeof(double)));
     HIP_CHECK(psi0.alloc(2 * nx * sizeof(double)));
-    HIP_CHECK(work.alloc(16 * (nx + 2) * sizeof(double)));
-    HIP_CHECK(job.alloc(sizeof(GsJob)));
-    GsJob J{};
+    std::vector<GsJob> jobs(1);
+    GsJob &J = jobs[0];
     J.num_r = num_r;
-    J.max_iter = kGsMaxIter;
     J.dx = dx;
     J.a = a;
     J.e_guess = e_guess;
     J.cv = cv;
     J.x = x.as<double>();
     J.psi0 = psi0.as<double>();
-    J.work = work.as<double>();
-    HIP_CHECK(hipMemcpy(job.p, &J, sizeof J, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(tdse_ground_state_kernel, dim3(1), dim3(64), 0, 0, job.as<GsJob>(), 1);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpy(&J, job.p, sizeof J, hipMemcpyDeviceToHost));
+    const int st = run_gs_jobs(jobs);
+    if (st != CTDSE_OK) return st;
     *e_out = J.energy;
     *iters = J.iters;
-    if (J.status == 1)
-        return set_status(CTDSE_ERR_GS_NOT_CONVERGED, "ground state: |dE| > CV after 1000 inverse iterations");
-    if (J.status == 2) return set_status(CTDSE_ERR_ZERO_PIVOT, "ground state: zero pivot in the Thomas solve");
-    return CTDSE_OK;
+    return gs_status(J, "");
 }
 
-int build_tables(ctdse_geometry *G)
+TableJob table_job(ctdse_geometry *G)
 {
-    const int nx = G->num_r + 1;
-    const int nchunks = (nx + TDSE_CHUNK - 1) / TDSE_CHUNK;
-    const long long npad = tdse_tiled_size(nchunks);
-    DevMem piv, job;
-    HIP_CHECK(G->g.alloc(npad * sizeof(double)));
-    HIP_CHECK(G->dpot.alloc(npad * sizeof(double)));
-    HIP_CHECK(G->ib.alloc(npad * sizeof(double2)));
-    HIP_CHECK(G->psi0.alloc(npad * sizeof(double2)));
-    HIP_CHECK(G->chq.alloc((nchunks + 128) * sizeof(double2)));
-    HIP_CHECK(G->chg.alloc((nchunks + 128) * sizeof(double2)));
-    HIP_CHECK(G->adq.alloc((nchunks + 128) * sizeof(double2)));
-    HIP_CHECK(G->adg.alloc((nchunks + 128) * sizeof(double2)));
-    HIP_CHECK(piv.alloc(2 * (size_t)nx * sizeof(double)));
-    HIP_CHECK(job.alloc(sizeof(TableJob)));
     TableJob J{};
     J.num_r = G->num_r;
     J.dx = G->dx;
@@ -151,25 +177,41 @@ int build_tables(ctdse_geometry *G)
     J.chg = G->chg.as<double2>();
     J.adq = G->adq.as<double2>();
     J.adg = G->adg.as<double2>();
-    J.pivot_work = piv.as<double>();
-    HIP_CHECK(hipMemcpy(job.p, &J, sizeof J, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(tdse_tables_grid_kernel, dim3(256), dim3(256), 0, 0, job.as<TableJob>());
-    hipLaunchKernelGGL(tdse_tables_pivot_kernel, dim3(1), dim3(64), 0, 0, job.as<TableJob>());
-    hipLaunchKernelGGL(tdse_tables_chunk_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, 0, job.as<TableJob>());
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpy(&J, job.p, sizeof J, hipMemcpyDeviceToHost));
+    return J;
+}
 
+int alloc_tables(ctdse_geometry *G)
+{
+    const int nx = G->num_r + 1;
+    const int nchunks = (nx + TDSE_CHUNK - 1) / TDSE_CHUNK;
+    const long long npad = tdse_tiled_size(nchunks);
+    HIP_CHECK(G->g.alloc(npad * sizeof(double)));
+    HIP_CHECK(G->dpot.alloc(npad * sizeof(double)));
+    HIP_CHECK(G->ib.alloc(npad * sizeof(double2)));
+    HIP_CHECK(G->psi0.alloc(npad * sizeof(double2)));
+    HIP_CHECK(G->chq.alloc((nchunks + 128) * sizeof(double2)));
+    HIP_CHECK(G->chg.alloc((nchunks + 128) * sizeof(double2)));
+    HIP_CHECK(G->adq.alloc((nchunks + 128) * sizeof(double2)));
+    HIP_CHECK(G->adg.alloc((nchunks + 128) * sizeof(double2)));
+    return CTDSE_OK;
+}
+
+// Host part of the table build once the kernels have run: truncation depth of the
+// segmented solve and the device descriptor.
+int finish_tables(ctdse_geometry *G, const TableJob &J, const std::string &who)
+{
+    const int nx = G->num_r + 1;
+    const int nchunks = (nx + TDSE_CHUNK - 1) / TDSE_CHUNK;
     const double rho = std::max(J.rho_fwd, J.rho_bwd);
     if (!(rho > 0.0) || !(rho < 0.999) || !std::isfinite(rho))
-        return set_status(CTDSE_ERR_UNSUPPORTED, "Crank-Nicolson factor decay rate " + std::to_string(rho) +
+        return set_status(CTDSE_ERR_UNSUPPORTED, who + "Crank-Nicolson factor decay rate " + std::to_string(rho) +
                                                      " too close to 1 for the segmented solve");
     const int W = (int)std::ceil(std::log(kTruncTol) / std::log(rho));
     const int H = (W + TDSE_CHUNK - 1) / TDSE_CHUNK;
     if (H > TDSE_MAX_HALO || TDSE_MAX_WAVES * 64 - H < 64)
-        return set_status(CTDSE_ERR_UNSUPPORTED, "truncation depth " + std::to_string(W) +
+        return set_status(CTDSE_ERR_UNSUPPORTED, who + "truncation depth " + std::to_string(W) +
                                                      " points exceeds the segment size (dt/dx^2 too large)");
-    if (J.pint_lo < 0 || J.pint_hi > G->num_r)
-        return set_status(CTDSE_ERR_ARGUMENT, "x_int lies outside the grid");
+    if (J.pint_lo < 0 || J.pint_hi > G->num_r) return set_status(CTDSE_ERR_ARGUMENT, who + "x_int lies outside the grid");
     TdseGeom &D = G->dev;
     D.g = G->g.as<double>();
     D.ib = G->ib.as<double2>();
@@ -181,7 +223,7 @@ int build_tables(ctdse_geometry *G)
     D.adg = G->adg.as<double2>();
     D.nx = nx;
     D.nchunks = nchunks;
-    D.npad = npad;
+    D.npad = tdse_tiled_size(nchunks);
     D.halo_lanes = H;
     D.pint_lo = J.pint_lo;
     D.pint_hi = J.pint_hi;
@@ -195,48 +237,195 @@ int build_tables(ctdse_geometry *G)
     return CTDSE_OK;
 }
 
+// Builds the factor tables of several geometries (x and psi0_nat set) in grouped launches;
+// note(index[k], code) is called (with the message set) for each geometry that fails.
+template <class Note>
+int build_tables(const std::vector<ctdse_geometry *> &gs, const std::vector<int> &index, Note note)
+{
+    size_t i0 = 0;
+    while (i0 < gs.size()) {
+        size_t i1 = i0, bytes = 0;
+        int max_chunks = 1;
+        while (i1 < gs.size() && (i1 - i0) < (size_t)kSetupGroupJobs) {
+            const size_t b = 2 * ((size_t)gs[i1]->num_r + 1) * sizeof(double);
+            if (i1 > i0 && bytes + b > kSetupScratchBytes) break;
+            bytes += b;
+            max_chunks = std::max(max_chunks, (gs[i1]->num_r + TDSE_CHUNK) / TDSE_CHUNK);
+            i1++;
+        }
+        const int m = (int)(i1 - i0);
+        DevMem piv, d_jobs;
+        HIP_CHECK(piv.alloc(bytes));
+        HIP_CHECK(d_jobs.alloc(m * sizeof(TableJob)));
+        std::vector<TableJob> jobs(m);
+        size_t off = 0;
+        for (int k = 0; k < m; k++) {
+            ctdse_geometry *G = gs[i0 + k];
+            if (alloc_tables(G) != CTDSE_OK) return g_status;
+            jobs[k] = table_job(G);
+            jobs[k].pivot_work = piv.as<double>() + off;
+            off += 2 * ((size_t)G->num_r + 1);
+        }
+        HIP_CHECK(hipMemcpy(d_jobs.p, jobs.data(), m * sizeof(TableJob), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(tdse_tables_grid_kernel, dim3(64, m), dim3(256), 0, 0, d_jobs.as<TableJob>());
+        hipLaunchKernelGGL(tdse_tables_pivot_kernel, dim3(m), dim3(64), 0, 0, d_jobs.as<TableJob>());
+        hipLaunchKernelGGL(tdse_tables_chunk_kernel, dim3((max_chunks + 255) / 256, m), dim3(256), 0, 0,
+                           d_jobs.as<TableJob>());
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpy(jobs.data(), d_jobs.p, m * sizeof(TableJob), hipMemcpyDeviceToHost));
+        for (int k = 0; k < m; k++) {
+            const int i = index[i0 + k];
+            const std::string who = gs.size() > 1 ? "geometry " + std::to_string(i) + ": " : "";
+            const int st = finish_tables(gs[i0 + k], jobs[k], who);
+            if (st != CTDSE_OK) note(i, st);
+        }
+        i0 = i1;
+    }
+    return CTDSE_OK;
+}
+
+struct GeomSpec {
+    int num_r;
+    double dx, dt, a, e_guess, cv, x_int;
+    const double *psi0_host;  // null: ground state by inverse iteration
+    const double *x_host;     // null: x_k = k dx - num_r dx / 2
+};
+
+int check_spec(const GeomSpec &g, const std::string &who)
+{
+    if (g.num_r < 2) return set_status(CTDSE_ERR_ARGUMENT, who + "num_r must be >= 2");
+    if (!(g.dx > 0) || !(g.dt > 0) || !(g.a > 0) || !std::isfinite(g.dx) || !std::isfinite(g.dt) || !std::isfinite(g.a))
+        return set_status(CTDSE_ERR_ARGUMENT, who + "dx, dt and the soft-Coulomb parameter a must be positive");
+    if (!(g.x_int > 0) || !std::isfinite(g.x_int)) return set_status(CTDSE_ERR_ARGUMENT, who + "x_int must be positive");
+    return CTDSE_OK;
+}
+
+// Builds geometries: ground states of all that need one in grouped launches, then the
+// tables.  out[i] is null where status[i] is a hard failure; a ground state that did not
+// converge (CTDSE_ERR_GS_NOT_CONVERGED) still yields a geometry, as in the reference, which
+// carries on with the last iterate.  Returns the first non-OK status (or OK).
+int geometries_from(const std::vector<GeomSpec> &spec, std::vector<ctdse_geometry *> &out, std::vector<int> &status)
+{
+    const int n = (int)spec.size();
+    out.assign(n, nullptr);
+    status.assign(n, CTDSE_OK);
+    if (n == 0) return CTDSE_OK;
+    if (device_ready() != CTDSE_OK) {
+        status.assign(n, g_status);
+        return g_status;
+    }
+    std::string first_msg;
+    int first = CTDSE_OK;
+    auto note = [&](int i, int code) {
+        status[i] = code;
+        if (first == CTDSE_OK && code != CTDSE_OK) {
+            first = code;
+            first_msg = g_message;
+        }
+    };
+    auto who = [&](int i) { return n > 1 ? "geometry " + std::to_string(i) + ": " : std::string(); };
+    std::vector<GsJob> jobs;
+    std::vector<int> job_of;
+    for (int i = 0; i < n; i++) {
+        const GeomSpec &g = spec[i];
+        if (check_spec(g, who(i)) != CTDSE_OK) {
+            note(i, g_status);
+            continue;
+        }
+        auto *G = new ctdse_geometry;
+        out[i] = G;
+        G->num_r = g.num_r;
+        G->dx = g.dx;
+        G->dt = g.dt;
+        G->a = g.a;
+        G->x_int = g.x_int;
+        const size_t nx = g.num_r + 1;
+        hipError_t e = G->x.alloc(nx * sizeof(double));
+        if (e == hipSuccess) e = G->psi0_nat.alloc(2 * nx * sizeof(double));
+        if (e == hipSuccess && g.psi0_host) {
+            e = hipMemcpy(G->psi0_nat.p, g.psi0_host, 2 * nx * sizeof(double), hipMemcpyHostToDevice);
+            if (e == hipSuccess && g.x_host) e = hipMemcpy(G->x.p, g.x_host, nx * sizeof(double), hipMemcpyHostToDevice);
+        }
+        if (e == hipSuccess && (!g.psi0_host || !g.x_host)) {
+            // the ground-state kernel writes x itself; a given psi0 without x gets the grid here
+            if (g.psi0_host) {
+                hipLaunchKernelGGL(tdse_xgrid_kernel, dim3((nx + 255) / 256), dim3(256), 0, 0, G->x.as<double>(),
+                                   g.num_r, g.dx);
+                e = hipGetLastError();
+            }
+        }
+        if (e != hipSuccess) {
+            set_status(CTDSE_ERR_DEVICE, who(i) + "geometry upload: " + hipGetErrorString(e));
+            note(i, g_status);
+            continue;
+        }
+        if (!g.psi0_host) {
+            GsJob J{};
+            J.num_r = g.num_r;
+            J.dx = g.dx;
+            J.a = g.a;
+            J.e_guess = g.e_guess;
+            J.cv = g.cv;
+            J.x = G->x.as<double>();
+            J.psi0 = G->psi0_nat.as<double>();
+            jobs.push_back(J);
+            job_of.push_back(i);
+        }
+    }
+    if (!jobs.empty()) {
+        if (run_gs_jobs(jobs) != CTDSE_OK) {
+            for (int i = 0; i < n; i++) {
+                delete out[i];
+                out[i] = nullptr;
+                status[i] = g_status;
+            }
+            return g_status;
+        }
+        for (size_t k = 0; k < jobs.size(); k++) {
+            ctdse_geometry *G = out[job_of[k]];
+            G->e_init = jobs[k].energy;
+            G->gs_iters = jobs[k].iters;
+            const int st = gs_status(jobs[k], who(job_of[k]));
+            if (st != CTDSE_OK) note(job_of[k], st);
+        }
+    }
+    std::vector<ctdse_geometry *> tab;
+    std::vector<int> tab_index;
+    for (int i = 0; i < n; i++)
+        if (out[i] && (status[i] == CTDSE_OK || status[i] == CTDSE_ERR_GS_NOT_CONVERGED)) {
+            tab.push_back(out[i]);
+            tab_index.push_back(i);
+        }
+    if (build_tables(tab, tab_index, note) != CTDSE_OK) {
+        for (int i = 0; i < n; i++) {
+            delete out[i];
+            out[i] = nullptr;
+            status[i] = g_status;
+        }
+        return g_status;
+    }
+    for (int i = 0; i < n; i++) {
+        if (out[i] && status[i] != CTDSE_OK && status[i] != CTDSE_ERR_GS_NOT_CONVERGED) {
+            delete out[i];
+            out[i] = nullptr;
+        }
+    }
+    if (first != CTDSE_OK) set_status(first, first_msg);
+    else clear_status();
+    return first;
+}
+
 int geometry_from(int num_r, double dx, double dt, double a, double e_guess, double cv, double x_int,
                   const double *psi0_host, const double *x_host, ctdse_geometry **out)
 {
     if (!out) return set_status(CTDSE_ERR_ARGUMENT, "null output pointer");
     *out = nullptr;
-    if (num_r < 2) return set_status(CTDSE_ERR_ARGUMENT, "num_r must be >= 2");
-    if (!(dx > 0) || !(dt > 0) || !(a > 0) || !std::isfinite(dx) || !std::isfinite(dt) || !std::isfinite(a))
-        return set_status(CTDSE_ERR_ARGUMENT, "dx, dt and the soft-Coulomb parameter a must be positive");
-    if (!(x_int > 0) || !std::isfinite(x_int)) return set_status(CTDSE_ERR_ARGUMENT, "x_int must be positive");
-    if (device_ready() != CTDSE_OK) return g_status;
-    auto *G = new ctdse_geometry;
-    G->num_r = num_r;
-    G->dx = dx;
-    G->dt = dt;
-    G->a = a;
-    G->x_int = x_int;
-    const size_t nx = num_r + 1;
-    int st = CTDSE_OK;
-    if (!psi0_host) {
-        st = run_ground_state(num_r, dx, a, e_guess, cv, G->x, G->psi0_nat, &G->e_init, &G->gs_iters);
-    } else {
-        hipError_t e = G->x.alloc(nx * sizeof(double));
-        if (e == hipSuccess) e = G->psi0_nat.alloc(2 * nx * sizeof(double));
-        if (e == hipSuccess) e = hipMemcpy(G->psi0_nat.p, psi0_host, 2 * nx * sizeof(double), hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            if (x_host) {
-                e = hipMemcpy(G->x.p, x_host, nx * sizeof(double), hipMemcpyHostToDevice);
-            } else {
-                hipLaunchKernelGGL(tdse_xgrid_kernel, dim3((nx + 255) / 256), dim3(256), 0, 0, G->x.as<double>(), num_r,
-                                   dx);
-                e = hipGetLastError();
-            }
-        }
-        if (e != hipSuccess) st = set_status(CTDSE_ERR_DEVICE, std::string("geometry upload: ") + hipGetErrorString(e));
-    }
-    if (st == CTDSE_OK) st = build_tables(G);
-    if (st != CTDSE_OK && st != CTDSE_ERR_GS_NOT_CONVERGED) {
-        delete G;
-        return st;
-    }
-    *out = G;
-    return st;
+    std::vector<GeomSpec> spec{{num_r, dx, dt, a, e_guess, cv, x_int, psi0_host, x_host}};
+    std::vector<ctdse_geometry *> gs;
+    std::vector<int> st;
+    const int code = geometries_from(spec, gs, st);
+    *out = gs[0];
+    return code;
 }
 
 struct InstSpec {
@@ -244,6 +433,7 @@ struct InstSpec {
     int n_steps;
     const double *field;     // device
     double *out;             // device, 5 rows of n_steps (may be null for partial runs)
+    long long out_row = 0;   // distance between the 5 rows (0: n_steps, packed)
     double *snap = nullptr;  // device
     int snap_every = 1, n_snap = 0;
     double *psi_final = nullptr;
@@ -273,81 +463,184 @@ int pick_waves(int *min_out, int max_halo_lanes)
     return nw;
 }
 
-int propagate(const std::vector<InstSpec> &spec, hipStream_t stream)
+// Launch shape of the propagation kernel: waves per workgroup (the segment is nw * 64
+// chunks) and resident workgroups per CU (the persistent grid is CUs * per_cu).
+struct LaunchShape {
+    int nw = 4, per_cu = 3;
+};
+
+int shape_for(const std::vector<const ctdse_geometry *> &geoms, const ctdse_launch_config *cfg, LaunchShape *out)
+{
+    int min_nw = 1, max_h = 0;
+    for (const ctdse_geometry *g : geoms) max_h = std::max(max_h, g->dev.halo_lanes);
+    LaunchShape sh;
+    sh.nw = pick_waves(&min_nw, max_h);
+    sh.per_cu = std::max(1, TDSE_WAVES_PER_CU / sh.nw);
+    if (const char *env = std::getenv("MMA_TDSE_WG_PER_CU")) sh.per_cu = std::max(1, std::atoi(env));
+    if (cfg) {
+        if (cfg->waves_per_group < min_nw || cfg->waves_per_group > TDSE_MAX_WAVES)
+            return set_status(CTDSE_ERR_ARGUMENT, "waves_per_group must lie in [" + std::to_string(min_nw) + ", " +
+                                                      std::to_string(TDSE_MAX_WAVES) + "] for these geometries");
+        if (cfg->groups_per_cu < 1 || cfg->groups_per_cu > 16)
+            return set_status(CTDSE_ERR_ARGUMENT, "groups_per_cu must lie in [1, 16]");
+        sh.nw = cfg->waves_per_group;
+        sh.per_cu = cfg->groups_per_cu;
+    }
+    *out = sh;
+    return CTDSE_OK;
+}
+
+} // namespace
+
+// A prepared batch: device descriptors, the instance queue and a private work buffer, so
+// that a launch is two kernel enqueues with no allocation, copy or synchronisation (and can
+// be captured into a graph).
+struct ctdse_batch_plan {
+    int n = 0, slots = 0;
+    LaunchShape shape;
+    long long stride = 0;
+    DevMem work, d_geom, d_inst, d_order, d_counter;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~ctdse_batch_plan()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace {
+
+// Descriptors of a batch on the host: validated instances, distinct geometries and the
+// longest-first queue order.
+struct BatchHost {
+    std::vector<const ctdse_geometry *> geoms;
+    std::vector<TdseGeom> gdev;
+    std::vector<TdseInst> insts;
+    std::vector<int> order;
+    long long stride = 0;
+};
+
+int describe(const std::vector<InstSpec> &spec, BatchHost &B)
 {
     const int n = (int)spec.size();
-    if (n == 0) return CTDSE_OK;
-    std::vector<const ctdse_geometry *> geoms;
-    std::vector<TdseInst> insts(n);
-    long long stride = 0;
+    B.insts.assign(n, TdseInst{});
     for (int i = 0; i < n; i++) {
         const InstSpec &s = spec[i];
         if (!s.geom || s.n_steps < 1 || !s.field || (!s.out && !s.save && s.n_run == 0) || s.k0 < 0 || s.n_run < 0 ||
             s.k0 + s.n_run > s.n_steps || (s.k0 > 0 && !s.pregated) || (s.save && s.save_every < 1))
             return set_status(CTDSE_ERR_ARGUMENT, "instance " + std::to_string(i) + ": bad descriptor");
-        int gi = (int)(std::find(geoms.begin(), geoms.end(), s.geom) - geoms.begin());
-        if (gi == (int)geoms.size()) geoms.push_back(s.geom);
-        TdseInst &I = insts[i];
+        int gi = (int)(std::find(B.geoms.begin(), B.geoms.end(), s.geom) - B.geoms.begin());
+        if (gi == (int)B.geoms.size()) B.geoms.push_back(s.geom);
+        TdseInst &I = B.insts[i];
         I.geom = gi;
         I.n_steps = s.n_steps;
         I.field = s.field;
+        const long long row = s.out_row > 0 ? s.out_row : s.n_steps;
+        if (s.out_row > 0 && s.out_row < s.n_steps)
+            return set_status(CTDSE_ERR_ARGUMENT, "instance " + std::to_string(i) + ": output rows shorter than n_steps");
         I.src = s.out;
-        I.ptot = s.out ? s.out + s.n_steps : nullptr;
-        I.pint = s.out ? s.out + 2LL * s.n_steps : nullptr;
-        I.expv = s.out ? s.out + 3LL * s.n_steps : nullptr;
-        I.efield = s.out ? s.out + 4LL * s.n_steps : nullptr;
+        I.ptot = s.out ? s.out + row : nullptr;
+        I.pint = s.out ? s.out + 2 * row : nullptr;
+        I.expv = s.out ? s.out + 3 * row : nullptr;
+        I.efield = s.out ? s.out + 4 * row : nullptr;
         I.k0 = s.k0;
         I.n_run = s.n_run;
         I.pregated = s.pregated;
         I.psi_init = s.psi_init;
         I.save = s.save;
         I.save_every = s.save_every;
+        // a state vector given as both start and save slot of a run of save_every steps is
+        // stepped in place
+        I.inplace = s.psi_init && (const void *)s.psi_init == (const void *)s.save && s.n_run == s.save_every;
         I.snap = s.snap;
         I.snap_every = std::max(1, s.snap_every);
         I.n_snap = s.n_snap;
         I.psi_final = s.psi_final;
-        stride = std::max(stride, s.geom->dev.npad);
+        B.stride = std::max(B.stride, s.geom->dev.npad);
     }
     // largest instances first (longest-processing-time order for the atomic queue)
-    std::vector<int> order(n);
-    for (int i = 0; i < n; i++) order[i] = i;
+    B.order.resize(n);
+    for (int i = 0; i < n; i++) B.order[i] = i;
     auto steps_of = [&](int p) { return (double)(spec[p].n_run > 0 ? spec[p].n_run : spec[p].n_steps - spec[p].k0); };
-    std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+    std::stable_sort(B.order.begin(), B.order.end(), [&](int p, int q) {
         return (double)spec[p].geom->dev.nx * steps_of(p) > (double)spec[q].geom->dev.nx * steps_of(q);
     });
-    std::vector<TdseGeom> gdev(geoms.size());
-    for (size_t k = 0; k < geoms.size(); k++) gdev[k] = geoms[k]->dev;
+    B.gdev.resize(B.geoms.size());
+    for (size_t k = 0; k < B.geoms.size(); k++) B.gdev[k] = B.geoms[k]->dev;
+    return CTDSE_OK;
+}
 
+int grid_slots(int n, const LaunchShape &sh, int *slots)
+{
     int dev = 0;
     HIP_CHECK(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    int min_nw = 1, max_h = 0;
-    for (const ctdse_geometry *g : geoms) max_h = std::max(max_h, g->dev.halo_lanes);
-    const int nw = pick_waves(&min_nw, max_h);
-    int per_cu = std::max(1, TDSE_WAVES_PER_CU / nw);
-    if (const char *env = std::getenv("MMA_TDSE_WG_PER_CU")) per_cu = std::max(1, std::atoi(env));
-    const int slots = std::max(1, std::min(n, prop.multiProcessorCount * per_cu));
+    int cus = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    *slots = std::max(1, std::min(n, cus * sh.per_cu));
+    return CTDSE_OK;
+}
+
+// Enqueues the queue reset and the persistent kernel (no allocation, copy or wait).
+int enqueue_plan(const ctdse_batch_plan &P, double2 *work, hipStream_t stream)
+{
+    hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, P.d_counter.as<int>(), 1, 0);
+    hipLaunchKernelGGL(tdse_propagate_kernel, dim3(P.slots), dim3(64 * P.shape.nw), 0, stream,
+                       P.d_geom.as<TdseGeom>(), P.d_inst.as<TdseInst>(), P.d_order.as<int>(), P.n,
+                       P.d_counter.as<int>(), work, P.stride);
+    HIP_CHECK(hipGetLastError());
+    return CTDSE_OK;
+}
+
+// Immediate path: descriptors copied on `stream`, the shared work buffer, and a wait.
+int propagate(const std::vector<InstSpec> &spec, hipStream_t stream, const ctdse_launch_config *cfg = nullptr)
+{
+    const int n = (int)spec.size();
+    if (n == 0) return CTDSE_OK;
+    BatchHost B;
+    if (describe(spec, B) != CTDSE_OK) return g_status;
+    ctdse_batch_plan P;
+    P.n = n;
+    P.stride = B.stride;
+    if (shape_for(B.geoms, cfg, &P.shape) != CTDSE_OK) return g_status;
+    if (grid_slots(n, P.shape, &P.slots) != CTDSE_OK) return g_status;
 
     std::lock_guard<std::mutex> lock(g_launch_mutex);
-    const size_t need = (size_t)slots * stride * sizeof(double2);
+    const size_t need = (size_t)P.slots * P.stride * sizeof(double2);
     if (need > g_work_bytes) {
         HIP_CHECK(g_work.alloc(need));
         g_work_bytes = need;
     }
-    DevMem d_geom, d_inst, d_order, d_counter;
-    HIP_CHECK(d_geom.alloc(gdev.size() * sizeof(TdseGeom)));
-    HIP_CHECK(d_inst.alloc(insts.size() * sizeof(TdseInst)));
-    HIP_CHECK(d_order.alloc(order.size() * sizeof(int)));
-    HIP_CHECK(d_counter.alloc(sizeof(int)));
-    HIP_CHECK(hipMemcpyAsync(d_geom.p, gdev.data(), gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(d_inst.p, insts.data(), insts.size() * sizeof(TdseInst), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(d_order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, d_counter.as<int>(), 1, 0);
-    hipLaunchKernelGGL(tdse_propagate_kernel, dim3(slots), dim3(64 * nw), 0, stream, d_geom.as<TdseGeom>(),
-                       d_inst.as<TdseInst>(), d_order.as<int>(), n, d_counter.as<int>(), g_work.as<double2>(), stride);
-    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(P.d_geom.alloc(B.gdev.size() * sizeof(TdseGeom)));
+    HIP_CHECK(P.d_inst.alloc(B.insts.size() * sizeof(TdseInst)));
+    HIP_CHECK(P.d_order.alloc(B.order.size() * sizeof(int)));
+    HIP_CHECK(P.d_counter.alloc(sizeof(int)));
+    HIP_CHECK(hipMemcpyAsync(P.d_geom.p, B.gdev.data(), B.gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(P.d_inst.p, B.insts.data(), B.insts.size() * sizeof(TdseInst), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(P.d_order.p, B.order.data(), B.order.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    if (enqueue_plan(P, g_work.as<double2>(), stream) != CTDSE_OK) return g_status;
     HIP_CHECK(hipStreamSynchronize(stream));
+    return CTDSE_OK;
+}
+
+// Plan path: everything allocated and uploaded here, once.
+int make_plan(const std::vector<InstSpec> &spec, const ctdse_launch_config *cfg, ctdse_batch_plan &P)
+{
+    const int n = (int)spec.size();
+    BatchHost B;
+    if (describe(spec, B) != CTDSE_OK) return g_status;
+    P.n = n;
+    P.stride = B.stride;
+    if (shape_for(B.geoms, cfg, &P.shape) != CTDSE_OK) return g_status;
+    if (grid_slots(std::max(n, 1), P.shape, &P.slots) != CTDSE_OK) return g_status;
+    HIP_CHECK(P.work.alloc((size_t)P.slots * P.stride * sizeof(double2)));
+    HIP_CHECK(P.d_geom.alloc(B.gdev.size() * sizeof(TdseGeom)));
+    HIP_CHECK(P.d_inst.alloc(B.insts.size() * sizeof(TdseInst)));
+    HIP_CHECK(P.d_order.alloc(B.order.size() * sizeof(int)));
+    HIP_CHECK(P.d_counter.alloc(sizeof(int)));
+    HIP_CHECK(hipMemcpy(P.d_geom.p, B.gdev.data(), B.gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(P.d_inst.p, B.insts.data(), B.insts.size() * sizeof(TdseInst), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(P.d_order.p, B.order.data(), B.order.size() * sizeof(int), hipMemcpyHostToDevice));
     return CTDSE_OK;
 }
 
@@ -537,6 +830,116 @@ int ctdse_propagate_batch(int n_inst, const ctdse_geometry *const *geom, const i
     }
     return propagate(spec, (hipStream_t)stream);
 }
+
+int ctdse_geometry_create_batch(int n, const int *num_r, const double *dx, const double *dt, const double *a,
+                                const double *e_guess, const double *cv, const double *x_int, ctdse_geometry **out,
+                                int *status)
+{
+    clear_status();
+    if (n < 0 || (n > 0 && (!num_r || !dx || !dt || !a || !e_guess || !cv || !x_int || !out)))
+        return set_status(CTDSE_ERR_ARGUMENT, "null geometry batch argument");
+    std::vector<GeomSpec> spec(n);
+    for (int i = 0; i < n; i++) spec[i] = GeomSpec{num_r[i], dx[i], dt[i], a[i], e_guess[i], cv[i], x_int[i], nullptr, nullptr};
+    std::vector<ctdse_geometry *> gs;
+    std::vector<int> st;
+    const int code = geometries_from(spec, gs, st);
+    for (int i = 0; i < n; i++) {
+        out[i] = gs.empty() ? nullptr : gs[i];
+        if (status) status[i] = st.empty() ? code : st[i];
+    }
+    return code;
+}
+
+int ctdse_launch_config_default(int n_inst, const ctdse_geometry *const *geom, ctdse_launch_config *cfg)
+{
+    clear_status();
+    if (n_inst < 1 || !geom || !cfg) return set_status(CTDSE_ERR_ARGUMENT, "null argument");
+    std::vector<const ctdse_geometry *> gs;
+    for (int i = 0; i < n_inst; i++) {
+        if (!geom[i]) return set_status(CTDSE_ERR_ARGUMENT, "null geometry " + std::to_string(i));
+        if (std::find(gs.begin(), gs.end(), geom[i]) == gs.end()) gs.push_back(geom[i]);
+    }
+    LaunchShape sh;
+    if (shape_for(gs, nullptr, &sh) != CTDSE_OK) return g_status;
+    cfg->waves_per_group = sh.nw;
+    cfg->groups_per_cu = sh.per_cu;
+    return CTDSE_OK;
+}
+
+int ctdse_batch_plan_create(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const double *field,
+                            const long long *field_off, double *out, const long long *out_off,
+                            long long out_row_stride, const ctdse_launch_config *config, ctdse_batch_plan **plan)
+{
+    clear_status();
+    if (!plan) return set_status(CTDSE_ERR_ARGUMENT, "null plan pointer");
+    *plan = nullptr;
+    if (n_inst < 1 || !geom || !n_steps || !field || !field_off || !out || !out_off)
+        return set_status(CTDSE_ERR_ARGUMENT, "null batch argument");
+    if (out_row_stride < 0) return set_status(CTDSE_ERR_ARGUMENT, "negative output row stride");
+    if (device_ready() != CTDSE_OK) return g_status;
+    std::vector<InstSpec> spec(n_inst);
+    for (int i = 0; i < n_inst; i++) {
+        spec[i].geom = geom[i];
+        spec[i].n_steps = n_steps[i];
+        spec[i].field = field + field_off[i];
+        spec[i].out = out + out_off[i];
+        spec[i].out_row = out_row_stride;
+    }
+    auto *P = new ctdse_batch_plan;
+    if (make_plan(spec, config, *P) != CTDSE_OK) {
+        delete P;
+        return g_status;
+    }
+    *plan = P;
+    return CTDSE_OK;
+}
+
+int ctdse_batch_plan_enqueue(ctdse_batch_plan *plan, void *stream)
+{
+    clear_status();
+    if (!plan) return set_status(CTDSE_ERR_ARGUMENT, "null plan");
+    return enqueue_plan(*plan, plan->work.as<double2>(), (hipStream_t)stream);
+}
+
+int ctdse_batch_plan_graph_launch(ctdse_batch_plan *plan, void *stream)
+{
+    clear_status();
+    if (!plan) return set_status(CTDSE_ERR_ARGUMENT, "null plan");
+    if (!plan->exec) {
+        hipStream_t cs = nullptr;
+        HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        int st = CTDSE_OK;
+        if (e == hipSuccess) st = enqueue_plan(*plan, plan->work.as<double2>(), cs);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(cs, &g);
+        (void)hipStreamDestroy(cs);
+        if (e != hipSuccess || e2 != hipSuccess || st != CTDSE_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            if (st != CTDSE_OK) return st;
+            return set_status(CTDSE_ERR_DEVICE, std::string("graph capture: ") +
+                                                    hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        plan->graph = g;
+        HIP_CHECK(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+    }
+    HIP_CHECK(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+    return CTDSE_OK;
+}
+
+int ctdse_batch_plan_info(const ctdse_batch_plan *plan, ctdse_launch_config *config, int *workgroups)
+{
+    clear_status();
+    if (!plan) return set_status(CTDSE_ERR_ARGUMENT, "null plan");
+    if (config) {
+        config->waves_per_group = plan->shape.nw;
+        config->groups_per_cu = plan->shape.per_cu;
+    }
+    if (workgroups) *workgroups = plan->slots;
+    return CTDSE_OK;
+}
+
+void ctdse_batch_plan_destroy(ctdse_batch_plan *plan) { delete plan; }
 
 long long ctdse_state_length(const ctdse_geometry *g)
 {

@@ -78,6 +78,7 @@ struct TdseInst {
     const double2 *psi_init;  // tiled start state (null: psi0)
     double2 *save;        // tiled states saved after every save_every-th step (may be null)
     int save_every;
+    int inplace;          // psi_init is stepped in place (no work copy, no saves)
 };
 
 // One instance of an adjoint (backward) sweep over steps k_hi .. k_lo+1, see tdse_adjoint_kernel.

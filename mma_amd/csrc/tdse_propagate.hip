@@ -299,7 +299,7 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
     const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    double2 *buf = work + (long long)blockIdx.x * work_stride;
+    double2 *const slot = work + (long long)blockIdx.x * work_stride;
 
     for (;;) {
         if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
@@ -313,9 +313,13 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
         const int own_chunks = nw * 64 - G.halo_lanes;  // host guarantees >= 64
         const int nseg = (G.nchunks + own_chunks - 1) / own_chunks;
 
-        // start state: psi0 (t = tmin) or a saved state; padding chunks are zero in both
-        const double2 *init = I.psi_init ? I.psi_init : G.psi0;
-        for (long long q = threadIdx.x; q < npad; q += blockDim.x) buf[q] = init[q];
+        // start state: psi0 (t = tmin) or a saved state; padding chunks are zero in both.  An
+        // in-place instance steps the caller's state vector itself.
+        double2 *const buf = I.inplace ? const_cast<double2 *>(I.psi_init) : slot;
+        if (!I.inplace) {
+            const double2 *init = I.psi_init ? I.psi_init : G.psi0;
+            for (long long q = threadIdx.x; q < npad; q += blockDim.x) buf[q] = init[q];
+        }
         const int k_start = I.k0;
         const int k_stop = I.n_run > 0 ? I.k0 + I.n_run : I.n_steps;
         if (threadIdx.x == 0 && I.src && k_start == 0) {
@@ -340,7 +344,7 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
                 F = fk;
             }
             double2 *save_to = nullptr;
-            if (I.save && (k - k_start + 1) % I.save_every == 0)
+            if (I.save && !I.inplace && (k - k_start + 1) % I.save_every == 0)
                 save_to = I.save + (long long)((k - k_start + 1) / I.save_every - 1) * npad;
             const double Fdt = F * G.dt;
             double rs, rc;

@@ -104,11 +104,13 @@ __device__ double energy(const double *psi, double c, const double *V, int n, do
 
 } // namespace
 
-// Inverse iteration for the ground state, tools.c:31-51 and tridiag.c:86-159; one lane per job.
+// Inverse iteration for the ground state, tools.c:31-51 and tridiag.c:86-159: one job per
+// workgroup, run by its first lane (the iteration is a serial chain; many geometries run
+// side by side, one per wave).
 __global__ void tdse_ground_state_kernel(GsJob *jobs, int n_jobs)
 {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n_jobs) return;
+    const int id = blockIdx.x;
+    if (id >= n_jobs || threadIdx.x != 0) return;
     GsJob &J = jobs[id];
     const int n = J.num_r;
     const long long sz = 2 * (long long)(n + 1);
@@ -171,10 +173,10 @@ __global__ void tdse_ground_state_kernel(GsJob *jobs, int n_jobs)
     J.status = status;
 }
 
-// Tables, part 1 (all threads of one block): g_j, V'(x_j) and psi0 in tiled order.
-__global__ void tdse_tables_grid_kernel(TableJob *job)
+// Tables, part 1 (blocks along x, one job per blockIdx.y): g_j, V'(x_j) and psi0 in tiled order.
+__global__ void tdse_tables_grid_kernel(TableJob *jobs)
 {
-    TableJob &J = *job;
+    TableJob &J = jobs[blockIdx.y];
     const int nx = J.num_r + 1;
     const long long nchunks = (nx + TDSE_CHUNK - 1) / TDSE_CHUNK;
     const long long total = tdse_tiled_size(nchunks);
@@ -208,10 +210,11 @@ __global__ void tdse_tables_grid_kernel(TableJob *job)
 // Tables, part 2 (one lane): pivots 1/beta_j of B' (tridiag.c:38-47 on the scaled
 // matrix), truncation decay rates, PopInt window (prop.c:86-93 with
 // tools_algorithmic.c:19-43) and the psi0 support window.
-__global__ void tdse_tables_pivot_kernel(TableJob *job)
+// One job per workgroup (blockIdx.x), first lane.
+__global__ void tdse_tables_pivot_kernel(TableJob *jobs)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    TableJob &J = *job;
+    if (threadIdx.x != 0) return;
+    TableJob &J = jobs[blockIdx.x];
     const int nx = J.num_r + 1;
     const double k36 = 36.0 * J.dt / (J.dx * J.dx);
     const double gk = (-3.0 * J.dt) / (J.dx * J.dx);
@@ -280,11 +283,11 @@ __global__ void tdse_tables_pivot_kernel(TableJob *job)
     J.p0_hi = hi;
 }
 
-// Tables, part 3 (one thread per chunk): transfer products of the two linear
-// recurrences of the factorised solve over each chunk.
-__global__ void tdse_tables_chunk_kernel(TableJob *job)
+// Tables, part 3 (one thread per chunk, one job per blockIdx.y): transfer products of the
+// two linear recurrences of the factorised solve over each chunk.
+__global__ void tdse_tables_chunk_kernel(TableJob *jobs)
 {
-    TableJob &J = *job;
+    TableJob &J = jobs[blockIdx.y];
     const int nx = J.num_r + 1;
     const int nchunks = (nx + TDSE_CHUNK - 1) / TDSE_CHUNK;
     const double gk = (-3.0 * J.dt) / (J.dx * J.dx);

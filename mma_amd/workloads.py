@@ -83,25 +83,51 @@ def gaussian_beam_batch(n_r=64, n_z=64, w0=100e-6, wavelength=800e-9, intensity=
     return t, fields
 
 
-def uneven_batch(n_inst=8192, seed=2024, dt=0.25, nr_range=(2000, 16000), cycles=(6, 20),
-                 intensity=(5e13, 3e14), wavelength=(800e-9, 1800e-9)):
-    """Uneven batch from numpy.random.Generator(PCG64(seed)): num_r = 2*U{nr_range},
-    cycles U{cycles}, intensity log-uniform, wavelength uniform; fields sampled at dt.
-    Returns a list of dicts with num_r, n_cycles, intensity, wavelength, t, field."""
+def _uneven_draw(n_inst, seed, dt, nr_range, cycles, intensity, wavelength):
     rng = np.random.Generator(np.random.PCG64(seed))
     half = rng.integers(nr_range[0], nr_range[1] + 1, size=n_inst)
     cyc = rng.integers(cycles[0], cycles[1] + 1, size=n_inst)
     inten = np.exp(rng.uniform(math.log(intensity[0]), math.log(intensity[1]), size=n_inst))
     lam = rng.uniform(wavelength[0], wavelength[1], size=n_inst)
+    n_e = np.array([int(math.ceil(cyc[i] * 2.0 * math.pi / omega_au(lam[i]) / dt)) + 1 for i in range(n_inst)])
+    return 2 * half, cyc, inten, lam, n_e
+
+
+UNEVEN = dict(n_inst=8192, seed=2024, dt=0.25, nr_range=(2000, 16000), cycles=(6, 20), intensity=(5e13, 3e14),
+              wavelength=(800e-9, 1800e-9))
+
+
+def uneven_batch(n_inst=8192, seed=2024, dt=0.25, nr_range=(2000, 16000), cycles=(6, 20),
+                 intensity=(5e13, 3e14), wavelength=(800e-9, 1800e-9), indices=None):
+    """Uneven batch from numpy.random.Generator(PCG64(seed)): num_r = 2*U{nr_range},
+    cycles U{cycles}, intensity log-uniform, wavelength uniform; fields sampled at dt.
+    Returns a list of dicts with num_r, n_cycles, intensity, wavelength, t, field (for the
+    members listed in `indices` only, if given; the draw is the same either way)."""
+    num_r, cyc, inten, lam, n_e = _uneven_draw(n_inst, seed, dt, nr_range, cycles, intensity, wavelength)
     out = []
-    for i in range(n_inst):
-        w = omega_au(lam[i])
-        T = cyc[i] * 2.0 * math.pi / w
-        n_e = int(math.ceil(T / dt)) + 1
-        t = np.arange(n_e) * dt
-        out.append(dict(num_r=int(2 * half[i]), n_cycles=int(cyc[i]), intensity=float(inten[i]),
-                        wavelength=float(lam[i]), t=t, field=sin2_pulse(t, field_au(inten[i]), w, int(cyc[i]))))
+    for i in (range(n_inst) if indices is None else indices):
+        t = np.arange(n_e[i]) * dt
+        out.append(dict(num_r=int(num_r[i]), n_cycles=int(cyc[i]), intensity=float(inten[i]),
+                        wavelength=float(lam[i]), t=t,
+                        field=sin2_pulse(t, field_au(inten[i]), omega_au(lam[i]), int(cyc[i]))))
     return out
+
+
+def uneven_work(n_inst=8192, seed=2024, dt=0.25, nr_range=(2000, 16000), cycles=(6, 20), intensity=(5e13, 3e14),
+                wavelength=(800e-9, 1800e-9)):
+    """(num_r, n_e) arrays of every member of uneven_batch, without building the fields."""
+    num_r, cyc, inten, lam, n_e = _uneven_draw(n_inst, seed, dt, nr_range, cycles, intensity, wavelength)
+    return num_r, n_e
+
+
+def stack_fields(fields, extra=3):
+    """Ragged field samples -> one (n, max_len + extra) array with the reference's zero tail
+    (FourInterp's calloc) and each member's step count (len + 2, the reference's outputs.Nt)."""
+    width = max(len(f) for f in fields) + extra
+    out = np.zeros((len(fields), width))
+    for i, f in enumerate(fields):
+        out[i, :len(f)] = f
+    return out, [len(f) + 2 for f in fields]
 
 
 # grid of the gradient workload: x in [-400, 400] a.u., dx = 0.1 (Nx = 8001), dt = 0.1 a.u.

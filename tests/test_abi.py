@@ -96,3 +96,35 @@ def test_batched_entry_points_reject_bad_arguments_before_touching_a_device():
     assert lib.ctdse_propagate_states(1, N, N, N, N, N, N, N, N, N, N, 1, N, N, N) == 3
     assert lib.ctdse_propagate_batch(1, N, N, N, N, N, N, N) == 3
     assert lib.ctdse_adjoint_reverse_batch(0, N, N, N, N, N, N, N, N, N, N, 1, 0, N, N, N) in (0, 5)  # empty batch
+
+
+def test_plan_and_geometry_batch_entry_points_reject_bad_arguments():
+    """The plan and batched-geometry entry points check their arguments before any device call."""
+    from mma_amd import library_path
+    lib = ctypes.CDLL(library_path())
+    lib.ctdse_last_error_message.restype = ctypes.c_char_p
+    N = None
+    plan = ctypes.c_void_p()
+    assert lib.ctdse_batch_plan_create(1, N, N, N, N, N, N, ctypes.c_longlong(0), N, ctypes.byref(plan)) == 3
+    assert plan.value is None
+    assert lib.ctdse_batch_plan_create(1, N, N, N, N, N, N, ctypes.c_longlong(0), N, N) == 3
+    assert lib.ctdse_batch_plan_enqueue(N, N) == 3
+    assert lib.ctdse_batch_plan_graph_launch(N, N) == 3
+    assert lib.ctdse_batch_plan_info(N, N, N) == 3
+    lib.ctdse_batch_plan_destroy(N)
+    assert lib.ctdse_geometry_create_batch(2, N, N, N, N, N, N, N, N, N) == 3
+    assert lib.ctdse_geometry_create_batch(-1, N, N, N, N, N, N, N, N, N) == 3
+    assert lib.ctdse_launch_config_default(0, N, N) == 3
+    assert b"null" in lib.ctdse_last_error_message()
+
+
+def test_autotune_candidates_and_signature():
+    from mma_amd import autotune
+    c = autotune.candidates(18)
+    assert (4, 3) in c and (8, 1) in c and (2, 6) in c
+    assert all(nw * 64 - 18 >= 64 and nw * per <= 12 for nw, per in c)
+    assert autotune.candidates(100)[0][0] == 3  # two-wave segments would hold fewer than 64 own chunks
+    k1, _ = autotune.signature("MI355X", 256, [16001] * 4096, [4416] * 4096)
+    k2, _ = autotune.signature("MI355X", 256, [16001] * 4096, [4416] * 4096)
+    k3, _ = autotune.signature("MI355X", 256, [16001] * 4095, [4416] * 4095)
+    assert k1 == k2 and k1 != k3

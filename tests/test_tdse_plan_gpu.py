@@ -1,0 +1,158 @@
+"""GPU checks of the prepared-batch path: geometries built many at a time, the plan's
+enqueue and its HIP-graph replay against the immediate entry point, the launch-shape
+autotuner, and members of the uneven workload (Nx, duration, intensity and wavelength drawn
+per instance) in one launch against the CPU oracle."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_err
+from mma_amd import autotune, tdse, workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-8
+ROWS = ["sourceterm", "PopTot", "PopInt", "expval", "Efield"]
+
+
+def _grid_geometry(num_r, dt=0.25):
+    return tdse.Geometry(num_r, 0.1, dt, W.GRID["a"], W.GRID["e_guess"], 1e-25, 2.0)
+
+
+def test_create_many_equals_one_at_a_time():
+    """Ground states run side by side give the same bits as separate launches."""
+    sizes = [1200, 2000, 3602, 800]
+    many = tdse.Geometry.create_many(sizes, 0.1, 0.25, W.GRID["a"], W.GRID["e_guess"], 1e-25, 2.0)
+    for n, g in zip(sizes, many):
+        one = _grid_geometry(n)
+        assert g.status == 0
+        assert g.e_init == one.e_init and g.gs_iterations == one.gs_iterations
+        assert g.halo_points == one.halo_points
+        xa, pa = g.ground_state()
+        xb, pb = one.ground_state()
+        np.testing.assert_array_equal(xa, xb)
+        np.testing.assert_array_equal(pa, pb)
+
+
+def test_create_many_reports_bad_members():
+    with pytest.raises(tdse.CTDSEError):
+        tdse.Geometry.create_many([1000, 1], 0.1, 0.25, W.GRID["a"])
+
+
+def test_plan_enqueue_and_graph_replay_match_immediate():
+    """The plan's enqueue and its graph give the immediate entry point's bits; replays are
+    interleaved with other device work and a refilled field buffer, and every replay is
+    checked against the eager result for the fields it read."""
+    t, fields = W.gaussian_beam_batch(n_r=4, n_z=3, n_cycles=2)
+    geo = _grid_geometry(3000)
+    dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
+    ref = tdse.propagate_batch(geo, dev).clone()
+    alt = torch.tensor(tdse.pad_fields(0.7 * fields[::-1].copy()), device="cuda")
+    ref_alt = tdse.propagate_batch(geo, alt).clone()
+
+    buf = dev.clone()
+    plan = tdse.BatchPlan(geo, buf)
+    out = plan.enqueue()
+    torch.cuda.synchronize()
+    torch.testing.assert_close(out, ref, rtol=0, atol=0)
+    for rep in range(4):
+        src = alt if rep % 2 else dev
+        want = ref_alt if rep % 2 else ref
+        buf.copy_(src)
+        plan.out.fill_(float("nan"))
+        junk = torch.randn(1 << 20, device="cuda").sum()  # unrelated eager work between replays
+        plan.replay()
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+        torch.testing.assert_close(plan.out, want, rtol=0, atol=0)
+    plan.close()
+
+
+def test_plan_rejects_bad_launch_shapes():
+    geo = _grid_geometry(1000)
+    dev = torch.zeros((2, 50), dtype=torch.float64, device="cuda")
+    with pytest.raises(tdse.CTDSEError):
+        tdse.BatchPlan(geo, dev, config=(9, 1))
+    with pytest.raises(tdse.CTDSEError):
+        tdse.BatchPlan(geo, dev, config=(4, 0))
+
+
+def test_launch_shapes_agree_with_oracle():
+    """Every launch shape the autotuner may choose gives the oracle's result (the segment
+    partition moves the truncation points, so agreement is to rounding, not bitwise)."""
+    from oracle import ctdse as O
+    t, fields = W.gaussian_beam_batch(n_r=2, n_z=2, n_cycles=2)
+    geo = _grid_geometry(4000)
+    dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
+    n_steps = dev.shape[1] - 1
+    x, psi0 = geo.ground_state()
+    ref = O.propagate(4000, 0.1, W.GRID["a"], 2.0, 0.25, psi0, x, tdse.pad_fields(fields[3])[0], n_steps)
+    for nw, per_cu in autotune.candidates(geo.halo_points // 8):
+        plan = tdse.BatchPlan(geo, dev, config=(nw, per_cu))
+        assert plan.config.as_tuple() == (nw, per_cu)
+        o = plan.enqueue().cpu().numpy()
+        torch.cuda.synchronize()
+        for r, k in enumerate(ROWS):
+            assert rel_err(o[3, r], ref[k][:n_steps]) < TOL, (nw, per_cu, k)
+        plan.close()
+
+
+def test_autotune_returns_a_timed_candidate(tmp_path, monkeypatch):
+    monkeypatch.setenv("MMA_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    t, fields = W.gaussian_beam_batch(n_r=8, n_z=8, n_cycles=2)
+    geo = _grid_geometry(4000)
+    dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
+    pairs = [(4, 3), (4, 2), (8, 1), (2, 6)]
+    res = autotune.tune(geo, dev, trial_steps=16, pairs=pairs)
+    assert not res.cached
+    assert res.config.as_tuple() in pairs
+    assert [r[:2] for r in res.table if r[:2] == res.config.as_tuple()]
+    assert all(r[2] > 0 for r in res.table) and res.table[0][2] == max(r[2] for r in res.table)
+    again = autotune.tune(geo, dev, trial_steps=16)
+    assert again.cached is False  # explicit pairs are not cached as the full search
+    res2 = autotune.tune(geo, dev, trial_steps=16)
+    assert res2.cached and res2.config.as_tuple() == again.config.as_tuple()
+    plan, _ = autotune.tuned_plan(geo, dev)
+    ref = tdse.propagate_batch(geo, dev)
+    o = plan.replay()
+    torch.cuda.synchronize()
+    assert float((o - ref).abs().max()) <= 1e-10 * float(ref.abs().max())
+
+
+def test_uneven_workload_members_match_oracle():
+    """Members of the seeded uneven workload (own Nx, cycles, intensity, wavelength) in one
+    launch through a tuned plan; the smallest are checked against the oracle and every
+    member conserves the norm."""
+    from oracle import ctdse as O
+    num_r, n_e = W.uneven_work()
+    work = (num_r + 1) * (n_e + 2)
+    pick = [int(i) for i in np.argsort(work)[:12]] + [int(np.argmax(work))]
+    sel = W.uneven_batch(indices=pick)
+    geoms = tdse.Geometry.create_many([m["num_r"] for m in sel], 0.1, 0.25, W.GRID["a"], W.GRID["e_guess"],
+                                      1e-25, 2.0)
+    fields, steps = W.stack_fields([m["field"] for m in sel])
+    dev = torch.tensor(fields, device="cuda")
+    plan = tdse.BatchPlan(geoms, dev, n_steps=steps)
+    out = plan.replay().cpu().numpy()
+    for i, s in enumerate(steps):  # instance i owns columns [0, steps[i])
+        assert np.all(out[i, 1, :s] <= 1.0 + 1e-12) and np.all(np.isfinite(out[i, :, :s])), pick[i]
+    for i in range(3):
+        m, g, s = sel[i], geoms[i], steps[i]
+        x, psi0 = g.ground_state()
+        ref = O.propagate(m["num_r"], 0.1, W.GRID["a"], 2.0, 0.25, psi0, x, fields[i], s)
+        for r, k in enumerate(ROWS):
+            e = rel_err(out[i, r, :s], ref[k][:s])
+            print(pick[i], m["num_r"], k, "%.2e" % e, flush=True)
+            assert e < TOL, (pick[i], k)
+    plan.close()
+
+
+def test_stepwise_form_equals_persistent_launch():
+    """One launch per time step (states stepped in place in HBM) gives the persistent
+    launch's bits."""
+    t, fields = W.gaussian_beam_batch(n_r=3, n_z=2, n_cycles=1)
+    geo = _grid_geometry(2400)
+    dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
+    ref = tdse.propagate_batch(geo, dev)
+    got = tdse.propagate_stepwise(geo, dev)
+    torch.cuda.synchronize()
+    torch.testing.assert_close(got, ref, rtol=0, atol=0)
