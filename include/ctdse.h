@@ -211,11 +211,14 @@ int ctdse_geometry_create_batch(int n, const int *num_r, const double *dx, const
                                 int *status);
 
 /* Launch shape of the persistent propagation kernel: waves per workgroup (one segment
- * of the parallel tridiagonal solve covers 64 * waves_per_group chunks of 8 points) and
- * resident workgroups per compute unit. */
+ * of the parallel tridiagonal solve covers 64 * waves_per_group chunks of 8 points),
+ * resident workgroups per compute unit, and time steps per pass over the state (1, or 2:
+ * the intermediate state stays on chip wherever no save or snapshot falls between the
+ * two steps). */
 typedef struct ctdse_launch_config {
     int waves_per_group;
     int groups_per_cu;
+    int steps_per_pass;
 } ctdse_launch_config;
 
 /* The shape ctdse_propagate_batch uses for these geometries. */

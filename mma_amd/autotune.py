@@ -1,13 +1,16 @@
 """Launch-shape autotuner of the persistent TDSE propagator.
 
-The propagation kernel's two launch parameters trade against each other:
+The propagation kernel's launch parameters trade against each other:
 
 * waves per workgroup sets the partition of the parallel tridiagonal solve: a segment is
   64 * waves chunks of 8 points, of which the truncation halo (H chunks, fixed by the
   geometry's decay rate) is recomputed by the next segment, so wider segments waste fewer
   lanes but synchronise more waves at every scan;
 * workgroups per compute unit sets how many instances share a CU and hide each other's
-  load and barrier stalls (the kernel's 168-register budget allows 12 resident waves per CU).
+  load and barrier stalls (the one-step kernel's 168-register budget allows 12 resident
+  waves per CU, the two-step kernel's 256-register budget 8);
+* steps per pass: the two-step kernel keeps the intermediate state on chip and streams the
+  state through HBM half as often, at lower occupancy.
 
 The best pair depends on the grid sizes, the halo and the batch size, so it is measured:
 every candidate runs the caller's own batch (same geometries, fields and queue order)
@@ -24,19 +27,21 @@ import torch
 from . import tdse
 
 MAX_WAVES = 8            # TDSE_MAX_WAVES of the kernel
-RESIDENT_WAVES_PER_CU = 12  # 3 waves per SIMD at the kernel's register budget
+RESIDENT_WAVES_PER_CU = {1: 12, 2: 8}  # waves per CU the register budget of each kernel allows
 CHUNK = 8
 
 
-def candidates(halo_lanes, max_waves=MAX_WAVES, resident=RESIDENT_WAVES_PER_CU):
-    """(waves_per_group, groups_per_cu) pairs worth timing: segments that keep at least 64
-    own chunks after the halo, and no more workgroups per CU than can be resident."""
+def candidates(halo_lanes, max_waves=MAX_WAVES, resident=RESIDENT_WAVES_PER_CU, steps=(1, 2)):
+    """(waves_per_group, groups_per_cu, steps_per_pass) shapes worth timing: segments that
+    keep at least 64 own chunks after the halo, and no more workgroups per CU than can be
+    resident."""
     out = []
-    for nw in (2, 3, 4, 5, 6, 8):
-        if nw > max_waves or nw * 64 - halo_lanes < 64:
-            continue
-        for per_cu in range(1, max(1, resident // nw) + 1):
-            out.append((nw, per_cu))
+    for sp in steps:
+        for nw in (2, 3, 4, 5, 6, 8):
+            if nw > max_waves or nw * 64 - halo_lanes < 64:
+                continue
+            for per_cu in range(1, max(1, resident[sp] // nw) + 1):
+                out.append((nw, per_cu, sp))
     return out
 
 
@@ -80,7 +85,7 @@ def _store_cache(key, entry):
 class TuneResult:
     def __init__(self, config, table, cached, key):
         self.config = config      # tdse.LaunchConfig
-        self.table = table        # [(waves, groups_per_cu, pt_steps_per_s)], fastest first
+        self.table = table        # [(waves, groups_per_cu, steps_per_pass, pt_steps_per_s)], fastest first
         self.cached = cached
         self.key = key
 
@@ -96,7 +101,8 @@ def tune(geometries, fields, n_steps=None, trial_steps=48, repeats=2, use_cache=
     n_steps:    the steps each instance will run (default L - 1).
     Each candidate runs min(n_steps[i], trial_steps) steps of every instance into a
     scratch output; the best of `repeats` timings counts.  pairs: time only these
-    (waves_per_group, groups_per_cu) shapes (neither read from nor written to the cache).
+    (waves_per_group, groups_per_cu, steps_per_pass) shapes (neither read from nor written to
+    the cache).
     Returns a TuneResult."""
     n_inst, L_f = fields.shape
     geoms = geometries if isinstance(geometries, (list, tuple)) else [geometries] * n_inst
@@ -112,13 +118,13 @@ def tune(geometries, fields, n_steps=None, trial_steps=48, repeats=2, use_cache=
     halo = max(g.halo_points // CHUNK for g in geoms)
     full_search = pairs is None
     pairs = list(pairs) if pairs is not None else candidates(halo)
-    trial = [max(1, min(int(s), int(trial_steps))) for s in n_steps]
+    trial = [max(1, min(int(s), int(trial_steps))) for s in n_steps]  # even: whole pairs
     width = max(trial)
     scratch = torch.empty((n_inst, 5, width), dtype=torch.float64, device=dev)
     stream = stream if stream is not None else torch.cuda.current_stream(dev)
     table = []
-    for nw, per_cu in pairs:
-        plan = tdse.BatchPlan(geoms, fields, n_steps=trial, out=scratch, config=(nw, per_cu))
+    for shape in pairs:
+        plan = tdse.BatchPlan(geoms, fields, n_steps=trial, out=scratch, config=shape)
         best = None
         with torch.cuda.stream(stream):
             plan.enqueue(stream)  # untimed first run
@@ -132,9 +138,9 @@ def tune(geometries, fields, n_steps=None, trial_steps=48, repeats=2, use_cache=
                 ms = t0.elapsed_time(t1)
                 best = ms if best is None else min(best, ms)
         plan.close()
-        table.append((nw, per_cu, plan.pt_steps / (best * 1e-3)))
-    table.sort(key=lambda r: -r[2])
-    cfg = tdse.LaunchConfig(table[0][0], table[0][1])
+        table.append(tuple(plan.config.as_tuple()) + (plan.pt_steps / (best * 1e-3),))
+    table.sort(key=lambda r: -r[-1])
+    cfg = tdse.LaunchConfig(*table[0][:3])
     if use_cache and full_search:
         _store_cache(key, {"config": list(cfg.as_tuple()), "table": [list(r) for r in table], "batch": blob})
     return TuneResult(cfg, table, False, key)

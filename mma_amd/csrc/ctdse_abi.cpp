@@ -73,6 +73,7 @@ DevMem g_work;
 size_t g_work_bytes = 0;
 
 constexpr int kGsMaxIter = 1000;      // tridiag.c:139
+constexpr int kDefaultStepsPerPass = 1;  // propagation kernel used when no launch shape is given
 constexpr double kTruncTol = 1e-17;   // relative truncation of the segmented solve
 
 } // namespace
@@ -466,7 +467,7 @@ int pick_waves(int *min_out, int max_halo_lanes)
 // Launch shape of the propagation kernel: waves per workgroup (the segment is nw * 64
 // chunks) and resident workgroups per CU (the persistent grid is CUs * per_cu).
 struct LaunchShape {
-    int nw = 4, per_cu = 3;
+    int nw = 4, per_cu = 3, steps = 1;
 };
 
 int shape_for(const std::vector<const ctdse_geometry *> &geoms, const ctdse_launch_config *cfg, LaunchShape *out)
@@ -475,7 +476,9 @@ int shape_for(const std::vector<const ctdse_geometry *> &geoms, const ctdse_laun
     for (const ctdse_geometry *g : geoms) max_h = std::max(max_h, g->dev.halo_lanes);
     LaunchShape sh;
     sh.nw = pick_waves(&min_nw, max_h);
-    sh.per_cu = std::max(1, TDSE_WAVES_PER_CU / sh.nw);
+    sh.steps = kDefaultStepsPerPass;
+    if (const char *env = std::getenv("MMA_TDSE_STEPS_PER_PASS")) sh.steps = std::atoi(env) == 2 ? 2 : 1;
+    sh.per_cu = std::max(1, (sh.steps == 2 ? 4 * TDSE_PAIR_WAVES_PER_SIMD : TDSE_WAVES_PER_CU) / sh.nw);
     if (const char *env = std::getenv("MMA_TDSE_WG_PER_CU")) sh.per_cu = std::max(1, std::atoi(env));
     if (cfg) {
         if (cfg->waves_per_group < min_nw || cfg->waves_per_group > TDSE_MAX_WAVES)
@@ -483,8 +486,11 @@ int shape_for(const std::vector<const ctdse_geometry *> &geoms, const ctdse_laun
                                                       std::to_string(TDSE_MAX_WAVES) + "] for these geometries");
         if (cfg->groups_per_cu < 1 || cfg->groups_per_cu > 16)
             return set_status(CTDSE_ERR_ARGUMENT, "groups_per_cu must lie in [1, 16]");
+        if (cfg->steps_per_pass != 1 && cfg->steps_per_pass != 2)
+            return set_status(CTDSE_ERR_ARGUMENT, "steps_per_pass must be 1 or 2");
         sh.nw = cfg->waves_per_group;
         sh.per_cu = cfg->groups_per_cu;
+        sh.steps = cfg->steps_per_pass;
     }
     *out = sh;
     return CTDSE_OK;
@@ -585,9 +591,9 @@ int grid_slots(int n, const LaunchShape &sh, int *slots)
 int enqueue_plan(const ctdse_batch_plan &P, double2 *work, hipStream_t stream)
 {
     hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, P.d_counter.as<int>(), 1, 0);
-    hipLaunchKernelGGL(tdse_propagate_kernel, dim3(P.slots), dim3(64 * P.shape.nw), 0, stream,
-                       P.d_geom.as<TdseGeom>(), P.d_inst.as<TdseInst>(), P.d_order.as<int>(), P.n,
-                       P.d_counter.as<int>(), work, P.stride);
+    hipLaunchKernelGGL(P.shape.steps == 2 ? tdse_propagate_pair_kernel : tdse_propagate_kernel, dim3(P.slots),
+                       dim3(64 * P.shape.nw), 0, stream, P.d_geom.as<TdseGeom>(), P.d_inst.as<TdseInst>(),
+                       P.d_order.as<int>(), P.n, P.d_counter.as<int>(), work, P.stride);
     HIP_CHECK(hipGetLastError());
     return CTDSE_OK;
 }
@@ -863,6 +869,7 @@ int ctdse_launch_config_default(int n_inst, const ctdse_geometry *const *geom, c
     if (shape_for(gs, nullptr, &sh) != CTDSE_OK) return g_status;
     cfg->waves_per_group = sh.nw;
     cfg->groups_per_cu = sh.per_cu;
+    cfg->steps_per_pass = sh.steps;
     return CTDSE_OK;
 }
 
@@ -934,6 +941,7 @@ int ctdse_batch_plan_info(const ctdse_batch_plan *plan, ctdse_launch_config *con
     if (config) {
         config->waves_per_group = plan->shape.nw;
         config->groups_per_cu = plan->shape.per_cu;
+        config->steps_per_pass = plan->shape.steps;
     }
     if (workgroups) *workgroups = plan->slots;
     return CTDSE_OK;

@@ -20,6 +20,7 @@
 #define TDSE_MAX_WAVES 8
 #define TDSE_WAVES_PER_SIMD 3      // propagation kernel: register budget 168 VGPRs, 12 waves per CU
 #define TDSE_WAVES_PER_CU (4 * TDSE_WAVES_PER_SIMD)
+#define TDSE_PAIR_WAVES_PER_SIMD 2 // two-step propagation kernel: 256-VGPR budget, 8 waves per CU
 #define TDSE_ADJ_WAVES_PER_CU 8    // adjoint kernel: 2 waves per SIMD (224 VGPRs)
 #define TDSE_REV_WAVES_PER_SIMD 2  // reconstructing adjoint kernel: 256-VGPR budget, 8 waves per CU
 #define TDSE_MAX_HALO 40           // largest truncation depth in chunks
@@ -137,6 +138,8 @@ __global__ void tdse_tables_pivot_kernel(TableJob *job);
 __global__ void tdse_tables_chunk_kernel(TableJob *job);
 __global__ void tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
                                       int *counter, double2 *work, long long work_stride);
+__global__ void tdse_propagate_pair_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
+                                           int *counter, double2 *work, long long work_stride);
 __global__ void tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *insts, const int *order, int n_inst,
                                     int *counter);
 __global__ void tdse_reverse_adjoint_kernel(const TdseGeom *geoms, const TdseRevInst *insts, const int *order,

@@ -94,16 +94,35 @@ __device__ inline void load_tables(const TdseGeom &G, int seg, int own_chunks, i
     T.gR = lane == 63 && gl < nwl * 64 - 1 ? ld1(G.g, tdse_chunk_base(min(c + 1, G.nchunks))) : 0.0;
 }
 
+// On-chip hand-off between the two steps of a pair pass (see tdse_propagate_kernel): the
+// first H chunks of the step-k+1 state of the segment just solved (the next stage's right
+// halo), and each wave's first and last point of it (neighbours across wave edges).
+struct PairLds {
+    double2 halo[2][TDSE_MAX_HALO][C];
+    double2 bnd[2][TDSE_MAX_WAVES][2];
+};
+
+// How a segment gets its state and where its result goes:
+//   kSingle: psi_k from HBM, psi_{k+1} stored to HBM (one step per pass);
+//   kKeep:   psi_k from HBM, psi_{k+1} kept in registers (R) and its first H chunks and
+//            wave-edge points published in LDS (first step of a pair);
+//   kFromRegs: psi_{k+1} from R (own chunks) and LDS (halo chunks, wave-edge neighbours),
+//            psi_{k+2} stored to HBM (second step of a pair).
+enum SegMode { kSingle = 0, kKeep = 1, kFromRegs = 2 };
+
 // Solve one segment of one step with all waves of the workgroup.
 // observe: add the observables of the state the step starts from (psi_k, read for the
 // solve anyway) to acc; the step's own result is observed by the next step.
 // Each lane keeps its chunk of psi and the chunk's coefficients (g_j, 1/beta_j) in
 // registers for the whole solve: all loads of a segment are issued together, once.
 // edge_in / edge_out: {old psi, forward z} at the last own point of the previous / this
-// segment (LDS, alternating between segments).
+// segment (LDS, alternating between segments).  par: parity of the segment (PairLds
+// buffers).
+template <int MODE>
 __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
                               double2 *__restrict__ psi, double2 e0, double2 rot, bool observe, Acc &acc,
-                              double2 (*slot_f)[2], double2 (*slot_b)[2], const double2 *edge_in, double2 *edge_out)
+                              double2 (*slot_f)[2], double2 (*slot_b)[2], const double2 *edge_in, double2 *edge_out,
+                              double2 (&R)[C], PairLds *pl)
 {
     const int gl = wave * 64 + lane;                 // lane index in the workgroup
     const int cs = seg * own_chunks;                 // first own chunk
@@ -121,16 +140,34 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     // table (padding row)
     const int cl = min(c, G.nchunks);
     const long long base = tdse_chunk_base(cl);
+    const int par = seg & 1;
 
     SegTables T;
     load_tables(G, seg, own_chunks, wave, lane, nw, T);
     double2 P[C];
-#pragma unroll
-    for (int i = 0; i < C; i++) P[i] = ld2(psi, base + 64 * i);
     const double2 zero2 = make_double2(0.0, 0.0);
     double2 nb = zero2;  // lane 0: the point left of the wave's first chunk; lane 63: right of its last
-    if (lane == 0 && gl > 0) nb = ld2(psi, tdse_chunk_base(c - 1) + 64 * (C - 1));  // not yet overwritten
-    if (lane == 63 && gl < nl - 1) nb = ld2(psi, tdse_chunk_base(min(c + 1, G.nchunks)));
+    if constexpr (MODE != kFromRegs) {
+#pragma unroll
+        for (int i = 0; i < C; i++) P[i] = ld2(psi, base + 64 * i);
+        if (lane == 0 && gl > 0) nb = ld2(psi, tdse_chunk_base(c - 1) + 64 * (C - 1));  // not yet overwritten
+        if (lane == 63 && gl < nl - 1) nb = ld2(psi, tdse_chunk_base(min(c + 1, G.nchunks)));
+    } else {
+        // own chunks: the previous stage's registers; halo chunks: its successor's first H
+        // chunks (LDS); past the grid: zero (the previous stage left those lanes unset)
+        const bool on_grid = c < G.nchunks;
+#pragma unroll
+        for (int i = 0; i < C; i++)
+            P[i] = !on_grid ? zero2 : (gl < own_chunks ? R[i] : pl->halo[par ^ 1][gl - own_chunks][i]);
+        if (lane == 0 && gl > 0 && c - 1 < G.nchunks) {
+            const int q = gl - 1;  // last chunk of the wave before
+            nb = q < own_chunks ? pl->bnd[par][wave - 1][1] : pl->halo[par ^ 1][q - own_chunks][C - 1];
+        }
+        if (lane == 63 && gl < nl - 1 && c + 1 < G.nchunks) {
+            const int q = gl + 1;  // first chunk of the wave after
+            nb = q < own_chunks ? pl->bnd[par][wave + 1][0] : pl->halo[par ^ 1][q - own_chunks][0];
+        }
+    }
     const bool own = gl < own_chunks && c < G.nchunks;
     const int j0 = c * C;
     // observables of the state this step starts from (psi_k)
@@ -221,12 +258,26 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     if (!own) return;  // only the segment's own chunks continue
 
     // length-gauge phase exp(+i F dt x_j) (prop.c:199-204) by rotation from the chunk start
-    // (e0), then store
+    // (e0), then store (or keep, for the second step of a pair)
     double2 e = e0;
+    if constexpr (MODE == kKeep) {
 #pragma unroll
-    for (int i = 0; i < C; i++) {
-        st2(psi, base + 64 * i, cmul(e, P[i]));
-        e = cmul(e, rot);
+        for (int i = 0; i < C; i++) {
+            R[i] = cmul(e, P[i]);
+            e = cmul(e, rot);
+        }
+        if (gl < G.halo_lanes) {
+#pragma unroll
+            for (int i = 0; i < C; i++) pl->halo[par][gl][i] = R[i];
+        }
+        if (lane == 0) pl->bnd[par][wave][0] = R[0];
+        if (lane == 63) pl->bnd[par][wave][1] = R[C - 1];
+    } else {
+#pragma unroll
+        for (int i = 0; i < C; i++) {
+            st2(psi, base + 64 * i, cmul(e, P[i]));
+            e = cmul(e, rot);
+        }
     }
 }
 
@@ -284,22 +335,55 @@ __device__ inline void store_observables(const TdseInst &I, const double (*part)
     if (I.efield) I.efield[idx] = F;
 }
 
-} // namespace
+// Field of step k after the gating of prop.c:125-134 (F = 0 up to and including the first
+// sign change of in_field); `gating` carries the state from step to step.
+__device__ inline double gated_field(const TdseInst &I, int k, bool &gating)
+{
+    const double fk = I.field[k];
+    if (gating) {
+        if (fk * I.field[k + 1] <= 0.0) gating = false;
+        return 0.0;
+    }
+    return fk;
+}
 
+// Phase factors of one step: per-point rotation, the phase at the first point of this lane's
+// chunk in segment 0, and the rotation from one segment to the next.
+struct StepPhase {
+    double2 rot, e_lane, r_seg;
+};
+__device__ inline StepPhase step_phase(const TdseGeom &G, double F, int wave, int lane, int own_chunks)
+{
+    StepPhase p;
+    const double Fdt = F * G.dt;
+    sincos(Fdt * G.dx, &p.rot.y, &p.rot.x);
+    const double x = __dsub_rn(__dmul_rn((double)((wave * 64 + lane) * C), G.dx), G.xmax);
+    sincos(Fdt * x, &p.e_lane.y, &p.e_lane.x);
+    sincos(Fdt * ((double)(own_chunks * C) * G.dx), &p.r_seg.y, &p.r_seg.x);
+    return p;
+}
+
+// The persistent propagation loop.  PAIR: whenever two steps k, k+1 follow each other with
+// no state save or snapshot between them, they run in one pass over psi: segment s of step
+// k is solved and kept in registers, then segment s-1 of step k+1 is solved from those
+// registers (its right halo is the first H chunks of segment s of step k+1, handed over in
+// LDS) and stored.  psi then streams through HBM once per two steps.
 // States saved for partial runs and snapshots are copied from the finished state between
 // steps (they are rare), so the segment solve carries no extra stores.
-__global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_WAVES_PER_SIMD)
-tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
-                      const int *__restrict__ order, int n_inst, int *counter, double2 *work, long long work_stride)
+template <bool PAIR>
+__device__ void propagate_loop(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
+                               const int *__restrict__ order, int n_inst, int *counter, double2 *work,
+                               long long work_stride, PairLds *pl)
 {
     __shared__ double part[2][TDSE_MAX_WAVES][5];
     __shared__ double2 slot_f[TDSE_MAX_WAVES][2], slot_b[TDSE_MAX_WAVES][2];
-    __shared__ double2 edge[2][2];
+    __shared__ double2 edge[2][2], edge_b[2][2];
     __shared__ int cur;
     const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     double2 *const slot = work + (long long)blockIdx.x * work_stride;
+    double2 R0[C], R1[C];  // pair passes: step-k+1 state of the last two segments solved
 
     for (;;) {
         if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
@@ -334,50 +418,77 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
         bool gating = !I.pregated;  // prop.c:125-134
         int i_snap = 0;
         double F_prev = 0.0;  // field of the previous step
-        for (int k = k_start; k < k_stop; k++) {
-            const double fk = I.field[k];
-            double F;
-            if (gating) {
-                if (fk * I.field[k + 1] <= 0.0) gating = false;
-                F = 0.0;
+        // a state save or a snapshot is due after step k
+        auto save_due = [&](int k) { return I.save && !I.inplace && (k - k_start + 1) % I.save_every == 0; };
+        auto snap_due = [&](int k) { return I.snap && k % I.snap_every == 0 && i_snap < I.n_snap - 1; };
+        for (int k = k_start; k < k_stop;) {
+            const bool pair = PAIR && k + 1 < k_stop && !save_due(k) && !snap_due(k);
+            if (pair) {
+                const double F0 = gated_field(I, k, gating);
+                const double F1 = gated_field(I, k + 1, gating);
+                const StepPhase ph0 = step_phase(G, F0, wave, lane, own_chunks);
+                const StepPhase ph1 = step_phase(G, F1, wave, lane, own_chunks);
+                const bool observe0 = k > k_start;
+                Acc acc0 = {0.0, 0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0, 0.0};
+                double2 e0 = ph0.e_lane, e1 = ph1.e_lane;
+                // segment s of step k into R(s&1), then segment s-1 of step k+1 from R((s-1)&1)
+                for (int s = 0; s <= nseg; s++) {
+                    if (s < nseg) {
+                        if (s & 1)
+                            solve_segment<kKeep>(G, s, own_chunks, wave, lane, nw, buf, e0, ph0.rot, observe0, acc0,
+                                                 slot_f, slot_b, edge[s & 1], edge[(s + 1) & 1], R1, pl);
+                        else
+                            solve_segment<kKeep>(G, s, own_chunks, wave, lane, nw, buf, e0, ph0.rot, observe0, acc0,
+                                                 slot_f, slot_b, edge[s & 1], edge[(s + 1) & 1], R0, pl);
+                        e0 = cmul(e0, ph0.r_seg);
+                    }
+                    lds_barrier();  // the halo and wave-edge points of segment s are in LDS
+                    if (s > 0) {
+                        const int sb = s - 1;
+                        if (sb & 1)
+                            solve_segment<kFromRegs>(G, sb, own_chunks, wave, lane, nw, buf, e1, ph1.rot, true, acc1,
+                                                     slot_f, slot_b, edge_b[sb & 1], edge_b[(sb + 1) & 1], R1, pl);
+                        else
+                            solve_segment<kFromRegs>(G, sb, own_chunks, wave, lane, nw, buf, e1, ph1.rot, true, acc1,
+                                                     slot_f, slot_b, edge_b[sb & 1], edge_b[(sb + 1) & 1], R0, pl);
+                        e1 = cmul(e1, ph1.r_seg);
+                    }
+                }
+                wave_observables(acc0, wave, lane, part[0]);
+                wave_observables(acc1, wave, lane, part[1]);
+                __syncthreads();  // also: psi of step k + 2 is complete in memory
+                if (threadIdx.x == 0 && I.src) {
+                    if (observe0) store_observables(I, part[0], nw, k, F_prev);
+                    store_observables(I, part[1], nw, k + 1, F0);
+                }
+                F_prev = F1;
+                k += 1;  // the save/snapshot test below is for step k + 1
             } else {
-                F = fk;
+                const double F = gated_field(I, k, gating);
+                const StepPhase ph = step_phase(G, F, wave, lane, own_chunks);
+                // observables of psi_k (entry k), except for the run's start state, whose entry is
+                // the initial values (k = 0) or was written by the run that ended there
+                const bool observe = k > k_start;
+                Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
+                double2 e0 = ph.e_lane;
+                for (int s = 0; s < nseg; s++) {
+                    solve_segment<kSingle>(G, s, own_chunks, wave, lane, nw, buf, e0, ph.rot, observe, acc, slot_f,
+                                           slot_b, edge[s & 1], edge[(s + 1) & 1], R0, pl);
+                    e0 = cmul(e0, ph.r_seg);
+                }
+                wave_observables(acc, wave, lane, part[k & 1]);
+                __syncthreads();  // also: psi of step k + 1 is complete in memory
+                if (threadIdx.x == 0 && I.src && observe) store_observables(I, part[k & 1], nw, k, F_prev);
+                F_prev = F;
             }
-            double2 *save_to = nullptr;
-            if (I.save && !I.inplace && (k - k_start + 1) % I.save_every == 0)
-                save_to = I.save + (long long)((k - k_start + 1) / I.save_every - 1) * npad;
-            const double Fdt = F * G.dt;
-            double rs, rc;
-            sincos(Fdt * G.dx, &rs, &rc);
-            const double2 rot = make_double2(rc, rs);
-            // the phase at the first point of this lane's chunk in segment 0, and its rotation
-            // from one segment to the next (own_chunks * TDSE_CHUNK points)
-            double2 e_lane, r_seg;
-            {
-                const double x = __dsub_rn(__dmul_rn((double)((wave * 64 + lane) * C), G.dx), G.xmax);
-                sincos(Fdt * x, &e_lane.y, &e_lane.x);
-                sincos(Fdt * ((double)(own_chunks * C) * G.dx), &r_seg.y, &r_seg.x);
-            }
-            // snapshots, prop.c:248-255: row i_snap gets psi after step k when k % every == 0
+            // state after step k: saves (partial runs) and snapshots (prop.c:248-255: row i_snap
+            // gets psi after step k when k % every == 0)
+            double2 *save_to = save_due(k) ? I.save + (long long)((k - k_start + 1) / I.save_every - 1) * npad : nullptr;
             double *snap_row = nullptr;
-            if (I.snap && k % I.snap_every == 0 && i_snap < I.n_snap - 1) {
+            if (snap_due(k)) {
                 snap_row = I.snap + (long long)i_snap * 2 * G.nx;
                 i_snap++;
             }
-            // observables of psi_k (entry k), except for the run's start state, whose entry is
-            // the initial values (k = 0) or was written by the run that ended there
-            const bool observe = k > k_start;
-            Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
-            double2 e0 = e_lane;
-            for (int s = 0; s < nseg; s++) {
-                solve_segment(G, s, own_chunks, wave, lane, nw, buf, e0, rot, observe, acc, slot_f, slot_b,
-                              edge[s & 1], edge[(s + 1) & 1]);
-                e0 = cmul(e0, r_seg);
-            }
-            wave_observables(acc, wave, lane, part[k & 1]);
-            __syncthreads();  // also: psi of step k + 1 is complete in memory
-            if (threadIdx.x == 0 && I.src && observe) store_observables(I, part[k & 1], nw, k, F_prev);
-            F_prev = F;
             if (save_to || snap_row) {
                 if (save_to)
                     for (long long q = threadIdx.x; q < npad; q += blockDim.x) save_to[q] = buf[q];
@@ -389,6 +500,7 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
                     }
                 __syncthreads();  // the copies have read psi before the next step overwrites it
             }
+            k++;
         }
         // the final state of a partial run: entry k_stop (a full run has no entry n_steps)
         if (k_stop > k_start && k_stop < I.n_steps) {
@@ -418,4 +530,24 @@ tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__rest
         }
         __syncthreads();
     }
+}
+
+} // namespace
+
+// One time step per pass over psi (168-VGPR budget, 12 waves per CU).
+__global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_WAVES_PER_SIMD)
+tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
+                      const int *__restrict__ order, int n_inst, int *counter, double2 *work, long long work_stride)
+{
+    propagate_loop<false>(geoms, insts, order, n_inst, counter, work, work_stride, nullptr);
+}
+
+// Two time steps per pass wherever no save or snapshot falls between them.
+__global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_PAIR_WAVES_PER_SIMD)
+tdse_propagate_pair_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
+                           const int *__restrict__ order, int n_inst, int *counter, double2 *work,
+                           long long work_stride)
+{
+    __shared__ PairLds pl;
+    propagate_loop<true>(geoms, insts, order, n_inst, counter, work, work_stride, &pl);
 }

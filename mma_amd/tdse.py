@@ -22,15 +22,18 @@ OUTPUT_ROWS = ("sourceterm", "PopTot", "PopInt", "expval", "Efield")
 
 class LaunchConfig(ctypes.Structure):
     """Launch shape of the persistent propagation kernel (ctdse_launch_config): waves per
-    workgroup (a segment of the parallel solve is 64 * waves_per_group chunks of 8 points)
-    and resident workgroups per compute unit."""
-    _fields_ = [("waves_per_group", c_int), ("groups_per_cu", c_int)]
+    workgroup (a segment of the parallel solve is 64 * waves_per_group chunks of 8 points),
+    resident workgroups per compute unit, and time steps per pass over the state (1 or 2)."""
+    _fields_ = [("waves_per_group", c_int), ("groups_per_cu", c_int), ("steps_per_pass", c_int)]
+
+    def __init__(self, waves_per_group=4, groups_per_cu=3, steps_per_pass=1):
+        super().__init__(int(waves_per_group), int(groups_per_cu), int(steps_per_pass))
 
     def as_tuple(self):
-        return (self.waves_per_group, self.groups_per_cu)
+        return (self.waves_per_group, self.groups_per_cu, self.steps_per_pass)
 
     def __repr__(self):
-        return "LaunchConfig(waves_per_group=%d, groups_per_cu=%d)" % self.as_tuple()
+        return "LaunchConfig(waves_per_group=%d, groups_per_cu=%d, steps_per_pass=%d)" % self.as_tuple()
 
 
 def _lib():
@@ -247,7 +250,8 @@ class BatchPlan:
     allocated once; enqueue() queues the persistent launch with no allocation, copy or wait
     (torch.cuda.graph can capture it) and replay() launches a HIP graph of the same work that
     the library captured on first use.  Arguments as propagate_batch; config is a
-    LaunchConfig or a (waves_per_group, groups_per_cu) pair (None: the library default).
+    LaunchConfig or a (waves_per_group, groups_per_cu[, steps_per_pass]) tuple (None: the
+    library default).
     Runs read `fields` and write `out` in place, so refilling `fields` and replaying
     propagates new fields.  Runs of one plan must not overlap."""
 

@@ -121,8 +121,8 @@ def test_plan_and_geometry_batch_entry_points_reject_bad_arguments():
 def test_autotune_candidates_and_signature():
     from mma_amd import autotune
     c = autotune.candidates(18)
-    assert (4, 3) in c and (8, 1) in c and (2, 6) in c
-    assert all(nw * 64 - 18 >= 64 and nw * per <= 12 for nw, per in c)
+    assert (4, 3, 1) in c and (8, 1, 1) in c and (2, 6, 1) in c and (4, 2, 2) in c and (4, 3, 2) not in c
+    assert all(nw * 64 - 18 >= 64 and nw * per <= (12 if sp == 1 else 8) for nw, per, sp in c)
     assert autotune.candidates(100)[0][0] == 3  # two-wave segments would hold fewer than 64 own chunks
     k1, _ = autotune.signature("MI355X", 256, [16001] * 4096, [4416] * 4096)
     k2, _ = autotune.signature("MI355X", 256, [16001] * 4096, [4416] * 4096)

@@ -71,9 +71,11 @@ def test_plan_rejects_bad_launch_shapes():
     geo = _grid_geometry(1000)
     dev = torch.zeros((2, 50), dtype=torch.float64, device="cuda")
     with pytest.raises(tdse.CTDSEError):
-        tdse.BatchPlan(geo, dev, config=(9, 1))
+        tdse.BatchPlan(geo, dev, config=(9, 1, 1))
     with pytest.raises(tdse.CTDSEError):
-        tdse.BatchPlan(geo, dev, config=(4, 0))
+        tdse.BatchPlan(geo, dev, config=(4, 0, 1))
+    with pytest.raises(tdse.CTDSEError):
+        tdse.BatchPlan(geo, dev, config=(4, 2, 3))
 
 
 def test_launch_shapes_agree_with_oracle():
@@ -86,13 +88,13 @@ def test_launch_shapes_agree_with_oracle():
     n_steps = dev.shape[1] - 1
     x, psi0 = geo.ground_state()
     ref = O.propagate(4000, 0.1, W.GRID["a"], 2.0, 0.25, psi0, x, tdse.pad_fields(fields[3])[0], n_steps)
-    for nw, per_cu in autotune.candidates(geo.halo_points // 8):
-        plan = tdse.BatchPlan(geo, dev, config=(nw, per_cu))
-        assert plan.config.as_tuple() == (nw, per_cu)
+    for shape in autotune.candidates(geo.halo_points // 8):
+        plan = tdse.BatchPlan(geo, dev, config=shape)
+        assert plan.config.as_tuple() == shape
         o = plan.enqueue().cpu().numpy()
         torch.cuda.synchronize()
         for r, k in enumerate(ROWS):
-            assert rel_err(o[3, r], ref[k][:n_steps]) < TOL, (nw, per_cu, k)
+            assert rel_err(o[3, r], ref[k][:n_steps]) < TOL, (shape, k)
         plan.close()
 
 
@@ -101,12 +103,12 @@ def test_autotune_returns_a_timed_candidate(tmp_path, monkeypatch):
     t, fields = W.gaussian_beam_batch(n_r=8, n_z=8, n_cycles=2)
     geo = _grid_geometry(4000)
     dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
-    pairs = [(4, 3), (4, 2), (8, 1), (2, 6)]
+    pairs = [(4, 3, 1), (4, 2, 1), (8, 1, 1), (2, 6, 1), (4, 2, 2)]
     res = autotune.tune(geo, dev, trial_steps=16, pairs=pairs)
     assert not res.cached
     assert res.config.as_tuple() in pairs
-    assert [r[:2] for r in res.table if r[:2] == res.config.as_tuple()]
-    assert all(r[2] > 0 for r in res.table) and res.table[0][2] == max(r[2] for r in res.table)
+    assert [r[:3] for r in res.table if r[:3] == res.config.as_tuple()]
+    assert all(r[3] > 0 for r in res.table) and res.table[0][3] == max(r[3] for r in res.table)
     again = autotune.tune(geo, dev, trial_steps=16)
     assert again.cached is False  # explicit pairs are not cached as the full search
     res2 = autotune.tune(geo, dev, trial_steps=16)
@@ -156,3 +158,68 @@ def test_stepwise_form_equals_persistent_launch():
     got = tdse.propagate_stepwise(geo, dev)
     torch.cuda.synchronize()
     torch.testing.assert_close(got, ref, rtol=0, atol=0)
+
+
+def _single_vs_pair(monkeypatch, run):
+    monkeypatch.setenv("MMA_TDSE_STEPS_PER_PASS", "1")
+    a = run()
+    monkeypatch.setenv("MMA_TDSE_STEPS_PER_PASS", "2")
+    b = run()
+    monkeypatch.delenv("MMA_TDSE_STEPS_PER_PASS")
+    return a, b
+
+
+def test_two_step_passes_give_single_step_bits(monkeypatch):
+    """Two steps per pass (the intermediate state kept on chip) give the bits of one step
+    per pass: the same segment arithmetic runs on the same values."""
+    t, fields = W.gaussian_beam_batch(n_r=3, n_z=3, n_cycles=2)
+    geo = _grid_geometry(5000)
+    dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
+    a, b = _single_vs_pair(monkeypatch, lambda: tdse.propagate_batch(geo, dev).clone())
+    torch.testing.assert_close(b, a, rtol=0, atol=0)
+    # odd and even step counts (a final single step), several geometries in one launch
+    geoms = tdse.Geometry.create_many([3000, 2001 * 2, 1600], 0.1, 0.25, W.GRID["a"], W.GRID["e_guess"], 1e-25, 2.0)
+    f3 = dev[:3].contiguous()
+    a, b = _single_vs_pair(monkeypatch,
+                           lambda: tdse.propagate_batch(geoms, f3, n_steps=[f3.shape[1] - 1, f3.shape[1] - 2, 301]).clone())
+    torch.testing.assert_close(b, a, rtol=0, atol=0)
+
+
+def test_two_step_passes_with_saves_partial_runs_and_snapshots(monkeypatch, dll):
+    """Saves after odd steps, partial runs of odd length from a saved state, and snapshots
+    every 81 steps force single steps in between pairs; every output agrees bit for bit."""
+    from mma_amd import adjoint as AD
+    from conftest import golden
+    t, fields = W.gaussian_beam_batch(n_r=2, n_z=2, n_cycles=2)
+    geo = _grid_geometry(3000)
+    dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
+    F = AD.gate(dev)[0].contiguous()
+    n = F.shape[1] - 1
+    S = AD.state_length(geo)
+
+    def partial():
+        save = torch.zeros((4, 9, S, 2), dtype=torch.float64, device="cuda")
+        out = torch.zeros((4, 5, n), dtype=torch.float64, device="cuda")
+        AD.run_states(geo, F, n, 0, 63, None, save, 7, out=out)
+        save2 = torch.zeros((4, 3, S, 2), dtype=torch.float64, device="cuda")
+        AD.run_states(geo, F, n, 63, 41, save, save2, 13, out=out, init_slot=8)
+        torch.cuda.synchronize()
+        return torch.cat([save.flatten(), save2.flatten(), out.flatten()])
+
+    a, b = _single_vs_pair(monkeypatch, partial)
+    torch.testing.assert_close(b, a, rtol=0, atol=0)
+
+    g = golden("ref_short_odd.npz")
+    from test_tdse_gpu import run_case
+
+    def snaps():
+        inputs, outputs = run_case(dll, g, writewft=1, tprint=20.25)
+        wf = outputs.get_wavefunction(inputs, grids=False)
+        res = np.concatenate([outputs.get_sourceterm(), outputs.get_PopTot(), wf.ravel().view(np.float64)])
+        dll.free_mtrx(outputs.psi, len(wf))
+        outputs.delete(dll)
+        inputs.delete(dll)
+        return res
+
+    a, b = _single_vs_pair(monkeypatch, snaps)
+    np.testing.assert_array_equal(a, b)
