@@ -255,7 +255,7 @@ def uneven_measure(steps=2):
     num_r = [m["num_r"] for m in members]
     del members
     t0 = time.perf_counter()
-    geoms = tdse.Geometry.create_many(num_r, 0.1, U["dt"], W.GRID["a"], W.GRID["e_guess"], W.GRID["cv"],
+    geoms = tdse.Geometry.create_many(num_r, 0.1, U["dt"], W.GRID["a"], W.GRID["e_guess"], W.UNEVEN_CV,
                                       W.GRID["x_int"])
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
@@ -291,7 +291,8 @@ def uneven_measure(steps=2):
             "max_over_min_instance_work": float(work.max() / work.min()),
             "value": pt / wall, "unit": "grid-pt-steps/s", "seconds": wall, "kernel_s": kern,
             "hbm_frac_psi_only": BYTES_PER_PT_STEP * pt / kern / 1e9 / HBM_PEAK_GBS,
-            "setup_s": t_setup, "tune_s": t_tune, "launch_config": list(res.config.as_tuple()),
+            "setup_s": t_setup, "gs_cv": W.UNEVEN_CV,
+            "gs_iterations_max": max(g.gs_iterations for g in geoms), "tune_s": t_tune, "launch_config": list(res.config.as_tuple()),
             "default_config": list(default.as_tuple()), "tune_table": [list(r) for r in res.table[:6]],
             "workgroups": plan.workgroups, "launch": "hipGraph replay of one persistent launch",
             "outputs_finite_and_norm_conserved": ok}

@@ -130,7 +130,7 @@ def test_uneven_workload_members_match_oracle():
     pick = [int(i) for i in np.argsort(work)[:12]] + [int(np.argmax(work))]
     sel = W.uneven_batch(indices=pick)
     geoms = tdse.Geometry.create_many([m["num_r"] for m in sel], 0.1, 0.25, W.GRID["a"], W.GRID["e_guess"],
-                                      1e-25, 2.0)
+                                      W.UNEVEN_CV, 2.0)
     fields, steps = W.stack_fields([m["field"] for m in sel])
     dev = torch.tensor(fields, device="cuda")
     plan = tdse.BatchPlan(geoms, dev, n_steps=steps)
