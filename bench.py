@@ -35,11 +35,27 @@ BYTES_PER_PT_STEP = 32.0  # psi read + write, complex128
 # HBM bytes per launch of tdse_propagate_kernel on the default 64 x 64 workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE counters scaled by their readings for a plain 1 GiB device copy
 # (x2.0 and x1.0); commands and readings in profiles/tdse_propagate_gaussian_beam_4096.md
-MEASURED_TRAFFIC_BYTES = 5.326e12 + 4.670e12
+MEASURED_TRAFFIC_BYTES = 5.371e12 + 4.671e12
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+class stdout_to_stderr:
+    """Sends file descriptor 1 to stderr for the duration (the reference's C code prints its
+    progress with printf; stdout carries only the result line)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
 
 
 def cpu_baseline(target_s=8.0):
@@ -241,7 +257,7 @@ def hankel_measure(cpu=True):
     return info
 
 
-def uneven_measure(steps=2):
+def uneven_measure(steps=2, config=None):
     """Uneven workload: 8192 instances drawn from a seeded PCG64 generator, each with its own
     grid (Nx in [4001, 32001]), duration (6-20 cycles), peak intensity (5e13-3e14 W/cm^2,
     log-uniform) and wavelength (800-1800 nm), dt = 0.25.  All ground states and tables are
@@ -264,7 +280,8 @@ def uneven_measure(steps=2):
     del fields
     out = torch.zeros((len(geoms), 5, max(n_steps)), dtype=torch.float64, device="cuda")
     t0 = time.perf_counter()
-    res = autotune.tune(geoms, dev, n_steps=n_steps)
+    res = autotune.tune(geoms, dev, n_steps=n_steps) if config is None else autotune.TuneResult(
+        tdse.LaunchConfig(*config), [], False, None)
     t_tune = time.perf_counter() - t0
     log("uneven: tuned %r in %.2f s (cached %s)" % (res.config, t_tune, res.cached))
     plan = tdse.BatchPlan(geoms, dev, n_steps=n_steps, out=out, config=res.config)
@@ -332,6 +349,9 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the 8192-instance uneven workload")
     ap.add_argument("--no-stepwise", action="store_true", help="skip the one-launch-per-step form")
     ap.add_argument("--only-uneven", action="store_true", help="time only the uneven workload")
+    ap.add_argument("--launch-config", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
+                    metavar="W,G,S", help="launch shape (waves/workgroup, workgroups/CU, steps/pass) instead "
+                                          "of the autotuner's (profiling runs: no trial launches)")
     ap.add_argument("--adj-inst", type=int, default=1024)
     ap.add_argument("--adj-samples", type=int, default=40000)
     ap.add_argument("--only-adjoint", action="store_true", help="time only the gradient workload")
@@ -350,10 +370,12 @@ def main():
         log("copied %d MiB three times" % args.copy_probe)
         return
     if args.only_hankel:
-        print(json.dumps({"hankel": hankel_measure(cpu=not args.no_cpu)}), flush=True)
+        with stdout_to_stderr():
+            hk = hankel_measure(cpu=not args.no_cpu)
+        print(json.dumps({"hankel": hk}), flush=True)
         return
     if args.only_uneven:
-        print(json.dumps({"uneven": uneven_measure()}), flush=True)
+        print(json.dumps({"uneven": uneven_measure(config=args.launch_config)}), flush=True)
         return
     if args.only_adjoint:
         import torch
@@ -380,7 +402,8 @@ def main():
     # launch shape from the autotuner (a few steps of every instance per candidate), then one
     # persistent launch per bench step, replayed from a HIP graph
     t_tune = time.perf_counter()
-    tuned = autotune.tune(geo, dev, n_steps=[n_steps] * n_inst)
+    tuned = autotune.tune(geo, dev, n_steps=[n_steps] * n_inst) if args.launch_config is None else \
+        autotune.TuneResult(tdse.LaunchConfig(*args.launch_config), [], False, None)
     t_tune = time.perf_counter() - t_tune
     log("launch shape %r (cached %s, %.2f s)" % (tuned.config, tuned.cached, t_tune))
     plan = tdse.BatchPlan(geo, dev, n_steps=[n_steps] * n_inst, out=out, config=tuned.config)
@@ -430,7 +453,7 @@ def main():
     if not args.no_uneven:
         out = None
         torch.cuda.empty_cache()
-        uneven = uneven_measure()
+        uneven = uneven_measure()  # tuned on its own (another batch signature)
         torch.cuda.empty_cache()
 
     adj = None
@@ -448,7 +471,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline()
+        with stdout_to_stderr():
+            cpu = cpu_baseline()
 
     line = {
         "metric": METRIC, "value": value, "unit": "grid-pt-steps/s", "n_gpus": 1, "steps": args.steps,

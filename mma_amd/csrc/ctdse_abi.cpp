@@ -577,6 +577,9 @@ int describe(const std::vector<InstSpec> &spec, BatchHost &B)
     return CTDSE_OK;
 }
 
+// Persistent workgroups of a launch: CUs * per_cu, or fewer for a smaller batch.  (Spreading
+// a batch over equal rounds instead, e.g. 683 workgroups for 4096 instances, measured slower:
+// 2.04 s against 1.95 s; the last, partly filled round runs faster per instance.)
 int grid_slots(int n, const LaunchShape &sh, int *slots)
 {
     int dev = 0;
