@@ -18,7 +18,9 @@
 #define TDSE_WAVE 64
 #define TDSE_TILE (TDSE_WAVE * TDSE_CHUNK)
 #define TDSE_MAX_WAVES 8
+#ifndef TDSE_WAVES_PER_SIMD
 #define TDSE_WAVES_PER_SIMD 3      // propagation kernel: register budget 168 VGPRs, 12 waves per CU
+#endif
 #define TDSE_WAVES_PER_CU (4 * TDSE_WAVES_PER_SIMD)
 #define TDSE_PAIR_WAVES_PER_SIMD 2 // two-step propagation kernel: 256-VGPR budget, 8 waves per CU
 #define TDSE_ADJ_WAVES_PER_CU 8    // adjoint kernel: 2 waves per SIMD (224 VGPRs)
