@@ -159,6 +159,19 @@ def spectrum(n, dt, tmax, signal):
     return omega, F, M2
 
 
+def gabor(signal, dt, n, n_freq, n_g, t_min, t_max, a):
+    """GaborTransform (tools_fftw3.c:158-213): rows i = 0..n_g-1 of
+    dt/sqrt(2pi) |FFT(exp(-((t_j - tau_i)/a)^2) s_j)|[0:n_freq], t_j = j dt,
+    tau_i = t_min + i (t_max - t_min)/n_g."""
+    s = np.asarray(signal, dtype=np.float64)[:n]
+    inv_a = 1.0 / a
+    dt_g = (t_max - t_min) / n_g
+    t = np.arange(n) * dt
+    tau = t_min + np.arange(n_g) * dt_g
+    rows = np.exp(-(inv_a * (t[None, :] - tau[:, None])) ** 2) * s[None, :]
+    return dt / np.sqrt(2.0 * np.pi) * np.abs(np.fft.rfft(rows, axis=1)[:, :n_freq])
+
+
 def refine(tgrid_in, dt, interp_by_dt_or_nt=0, nt_interp=1):
     """k1 rule of singleTDSE.c:55-68: returns (k1, refined dt, field dt)."""
     field_dt = tgrid_in[1] - tgrid_in[0]
