@@ -357,9 +357,38 @@ def main():
     ap.add_argument("--only-adjoint", action="store_true", help="time only the gradient workload")
     ap.add_argument("--no-hankel", action="store_true")
     ap.add_argument("--only-hankel", action="store_true", help="time only the far-field workload")
+    ap.add_argument("--stream-probe", type=int, default=0, metavar="GIB",
+                    help="time the library's streaming copy of GIB GiB (read + write ceiling) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB three times and exit (HBM counter calibration)")
     args = ap.parse_args()
+    if args.stream_probe:
+        import ctypes
+        import torch
+        from mma_amd import load_library
+        L = load_library()
+        L.ctdse_stream_copy.restype = ctypes.c_int
+        L.ctdse_stream_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int,
+                                        ctypes.c_void_p]
+        nbytes = args.stream_probe << 30
+        a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        st = torch.cuda.current_stream()
+        res = {}
+        for g in (4, 8, 16, 32):
+            assert L.ctdse_stream_copy(a.data_ptr(), b.data_ptr(), nbytes, g, st.cuda_stream) == 0
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(5):
+                L.ctdse_stream_copy(a.data_ptr(), b.data_ptr(), nbytes, g, st.cuda_stream)
+            e1.record(st)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / 5
+            res[g] = 2.0 * nbytes / (ms * 1e-3) / 1e12
+            log("stream copy %d GiB, %d workgroups/CU: %.3f ms, %.3f TB/s read + write" % (args.stream_probe, g, ms, res[g]))
+        assert torch.equal(a, b)
+        print(json.dumps({"stream_copy_TBps": res, "GiB": args.stream_probe}), flush=True)
+        return
     if args.copy_probe:
         import torch
         a = torch.ones(args.copy_probe << 17, dtype=torch.float64, device="cuda")

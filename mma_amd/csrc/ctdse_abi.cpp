@@ -952,6 +952,21 @@ int ctdse_batch_plan_info(const ctdse_batch_plan *plan, ctdse_launch_config *con
 
 void ctdse_batch_plan_destroy(ctdse_batch_plan *plan) { delete plan; }
 
+int ctdse_stream_copy(const double *src, double *dst, long long n_bytes, int groups_per_cu, void *stream)
+{
+    clear_status();
+    if (!src || !dst || n_bytes < 16 || n_bytes % 16 != 0 || groups_per_cu < 1 || groups_per_cu > 64)
+        return set_status(CTDSE_ERR_ARGUMENT, "stream copy: n_bytes a positive multiple of 16, 1..64 groups per CU");
+    if (device_ready() != CTDSE_OK) return g_status;
+    int dev = 0, cus = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL(tdse_stream_copy_kernel, dim3(cus * groups_per_cu), dim3(256), 0, (hipStream_t)stream,
+                       (const double2 *)src, (double2 *)dst, n_bytes / 16);
+    HIP_CHECK(hipGetLastError());
+    return CTDSE_OK;
+}
+
 long long ctdse_state_length(const ctdse_geometry *g)
 {
     clear_status();

@@ -326,6 +326,22 @@ __global__ void tdse_xgrid_kernel(double *x, int num_r, double dx)
     x[k] = (double)k * dx - half_span;
 }
 
+// Streaming copy dst[i] = src[i] in 16-byte lanes, four in flight per lane, grid-stride: the
+// read-and-write bandwidth probe the propagation kernel's traffic is compared with.
+__global__ void tdse_stream_copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n)
+{
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const double2 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void tdse_fill_int_kernel(int *p, int n, int v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
