@@ -336,6 +336,35 @@ def stepwise_measure(geo, dev, n_steps, ref):
     return wall, diff
 
 
+def stream_copy_rate(gib, groups_per_cu, reps=5):
+    """Read + write rate (TB/s) of the library's streaming copy (ctdse_stream_copy) over gib GiB,
+    HIP events on the launch stream: the practical ceiling the propagation traffic is compared
+    with."""
+    import ctypes
+    import torch
+    from mma_amd import load_library
+    L = load_library()
+    L.ctdse_stream_copy.restype = ctypes.c_int
+    L.ctdse_stream_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p]
+    nbytes = int(gib) << 30
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    st = torch.cuda.current_stream()
+    assert L.ctdse_stream_copy(a.data_ptr(), b.data_ptr(), nbytes, groups_per_cu, st.cuda_stream) == 0
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        L.ctdse_stream_copy(a.data_ptr(), b.data_ptr(), nbytes, groups_per_cu, st.cuda_stream)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    assert torch.equal(a, b)
+    rate = 2.0 * nbytes / (ms * 1e-3) / 1e12
+    log("stream copy %d GiB, %d workgroups/CU: %.3f ms, %.3f TB/s read + write" % (gib, groups_per_cu, ms, rate))
+    del a, b
+    return rate
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,30 +392,7 @@ def main():
                     help="run a plain device copy of MIB MiB three times and exit (HBM counter calibration)")
     args = ap.parse_args()
     if args.stream_probe:
-        import ctypes
-        import torch
-        from mma_amd import load_library
-        L = load_library()
-        L.ctdse_stream_copy.restype = ctypes.c_int
-        L.ctdse_stream_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int,
-                                        ctypes.c_void_p]
-        nbytes = args.stream_probe << 30
-        a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        st = torch.cuda.current_stream()
-        res = {}
-        for g in (4, 8, 16, 32):
-            assert L.ctdse_stream_copy(a.data_ptr(), b.data_ptr(), nbytes, g, st.cuda_stream) == 0
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(st)
-            for _ in range(5):
-                L.ctdse_stream_copy(a.data_ptr(), b.data_ptr(), nbytes, g, st.cuda_stream)
-            e1.record(st)
-            e1.synchronize()
-            ms = e0.elapsed_time(e1) / 5
-            res[g] = 2.0 * nbytes / (ms * 1e-3) / 1e12
-            log("stream copy %d GiB, %d workgroups/CU: %.3f ms, %.3f TB/s read + write" % (args.stream_probe, g, ms, res[g]))
-        assert torch.equal(a, b)
+        res = {g: stream_copy_rate(args.stream_probe, g) for g in (4, 8, 16, 32)}
         print(json.dumps({"stream_copy_TBps": res, "GiB": args.stream_probe}), flush=True)
         return
     if args.copy_probe:
@@ -497,6 +503,9 @@ def main():
         hk = hankel_measure(cpu=not args.no_cpu)
         torch.cuda.empty_cache()
 
+    copy_gbs = 1e3 * stream_copy_rate(16, 32)  # the box's read + write ceiling, same units as achieved
+    torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -512,6 +521,7 @@ def main():
                    "field": "800nm 10-cycle sin2, 1.5e14 W/cm2 peak, w0=100um", "precision": "complex128"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
+                     "copy_ceiling": copy_gbs, "frac_of_copy_ceiling": achieved / copy_gbs if copy_gbs else None,
                      "traffic": MEASURED_TRAFFIC_BYTES if (args.n_r, args.n_z) == (64, 64) else None},
         "cpu_baseline": cpu,
         "launch": {"form": "hipGraph replay of one persistent launch", "config": list(tuned.config.as_tuple()),
