@@ -58,13 +58,15 @@ class stdout_to_stderr:
         return False
 
 
-def cpu_baseline(target_s=8.0):
+def cpu_baseline(target_s=8.0, gpu_rows=None):
     """The CPU path on the host cores, on a bounded sample of the same workload: whole
     Gaussian-beam members (all steps, full grid), one member per thread.  When the reference's
     own CTDSE sources were compiled into oracle/_ref (gcc -O2), that build is timed (kind
     "reference", one call1DTDSE-equivalent per member from a thread pool; the reference keeps no
     mutable globals); otherwise the FP64 C oracle with OpenMP (kind "port").  The member count
-    is a multiple of the thread count chosen from a short one-thread calibration."""
+    is a multiple of the thread count chosen from a short one-thread calibration.  gpu_rows
+    (n_inst, 4, n_out): the GPU's sourceterm, PopTot, PopInt, expval of the same batch; the CPU
+    members are compared with them (max |a - b| / max |b| per row, and the L2 ratio)."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import ctdse as O
     from mma_amd import workloads as W
@@ -91,18 +93,32 @@ def cpu_baseline(target_s=8.0):
     t0 = time.perf_counter()
     if use_ref:
         with ThreadPoolExecutor(max_workers=threads) as pool:
-            list(pool.map(lambda i: O.ref_call(g["num_r"], g["dx"], g["a"], g["x_int"], 0.25, psi0, x, t, fields[i]),
-                          pick))
+            res = list(pool.map(lambda i: O.ref_call(g["num_r"], g["dx"], g["a"], g["x_int"], 0.25, psi0, x, t,
+                                                     fields[i]), pick))
+        cpu_rows = [np.stack([r[k][:n_out] for k in ("sourceterm", "PopTot", "PopInt", "expval")]) for r in res]
     else:
-        O.propagate_batch(g["num_r"], g["dx"], g["a"], g["x_int"], 0.25, psi0, x, padded[pick], n_out - 1,
-                          n_threads=threads)
+        res = O.propagate_batch(g["num_r"], g["dx"], g["a"], g["x_int"], 0.25, psi0, x, padded[pick], n_out - 1,
+                                n_threads=threads)
+        cpu_rows = None
     dt = time.perf_counter() - t0
+    parity = None
+    if gpu_rows is not None and cpu_rows is not None:
+        worst = np.zeros(4)
+        l2 = np.zeros(4)
+        for i, c in zip(pick, cpu_rows):
+            gr = gpu_rows[i, :, :n_out]
+            worst = np.maximum(worst, np.max(np.abs(gr - c), axis=1) / np.max(np.abs(c), axis=1))
+            l2 = np.maximum(l2, np.linalg.norm(gr - c, axis=1) / np.linalg.norm(c, axis=1))
+        parity = {"members": int(len(pick)), "against": "reference CTDSE sources on the CPU",
+                  "max_abs_over_max": dict(zip(("sourceterm", "PopTot", "PopInt", "expval"), worst.tolist())),
+                  "l2_rel": dict(zip(("sourceterm", "PopTot", "PopInt", "expval"), l2.tolist())),
+                  "within_1e-8": bool(np.all(worst <= 1e-8))}
     what = ("reference CTDSE sources (prop.c, tridiag.c, tools.c; gcc -O2 -ffp-contract=off), thread pool"
             if use_ref else "FP64 C oracle (gcc -O2 -ffp-contract=off), OpenMP")
     return {"value": n_inst * nx * n_out / dt, "unit": "grid-pt-steps/s", "cores": threads,
             "kind": "reference" if use_ref else "port",
             "sample": "%d of the 4096 Gaussian-beam members, %d samples each (Nx=%d), %s, one member per "
-                      "thread, %.1f s wall" % (n_inst, n_out, nx, what, dt)}
+                      "thread, %.1f s wall" % (n_inst, n_out, nx, what, dt)}, parity
 
 
 def adjoint_measure(n_inst, n_t):
@@ -460,8 +476,8 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     value = pt_steps / (wall / args.steps)
     achieved = BYTES_PER_PT_STEP * pt_steps / (kern_ms * 1e-3) / 1e9
-    src = out[:, 0, :].cpu().numpy()
-    finite = bool(np.all(np.isfinite(src)))
+    gpu_rows = out[:, :4, :].cpu().numpy()  # for the parity check against the CPU reference members
+    finite = bool(np.all(np.isfinite(gpu_rows)))
 
     stepwise = None
     if not args.no_stepwise:
@@ -506,11 +522,11 @@ def main():
     copy_gbs = 1e3 * stream_copy_rate(16, 32)  # the box's read + write ceiling, same units as achieved
     torch.cuda.empty_cache()
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
         log("cpu baseline")
         with stdout_to_stderr():
-            cpu = cpu_baseline()
+            cpu, parity = cpu_baseline(gpu_rows=gpu_rows)
 
     line = {
         "metric": METRIC, "value": value, "unit": "grid-pt-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -524,6 +540,7 @@ def main():
                      "copy_ceiling": copy_gbs, "frac_of_copy_ceiling": achieved / copy_gbs if copy_gbs else None,
                      "traffic": MEASURED_TRAFFIC_BYTES if (args.n_r, args.n_z) == (64, 64) else None},
         "cpu_baseline": cpu,
+        "parity_vs_reference": parity,
         "launch": {"form": "hipGraph replay of one persistent launch", "config": list(tuned.config.as_tuple()),
                    "tune_table": [list(r) for r in tuned.table[:6]], "tune_s": t_tune, "cached": tuned.cached},
         "stepwise_form": stepwise,
