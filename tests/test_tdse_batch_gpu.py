@@ -80,3 +80,25 @@ def test_mixed_geometries_and_lengths_match_oracle():
         o = o.cpu().numpy()
         for r, k in enumerate(ROWS):
             assert rel_err(o[r], ref[k][:s]) < TOL, (num_r, k)
+
+
+def test_full_gaussian_beam_batch_invariants():
+    """The whole 64 x 64 Gaussian-beam batch (Nx 16001, 4416 steps) through a graph-replayed
+    plan: every member conserves the norm (PopTot <= 1) and stays finite, and members run
+    alone or in a small batch give bitwise the rows of the full batch (the result of an
+    instance does not depend on the batch around it or on which workgroup ran it)."""
+    t, fields = W.gaussian_beam_batch(n_r=64, n_z=64)
+    geo = tdse.Geometry(W.GRID["num_r"], W.GRID["dx"], 0.25, W.GRID["a"], W.GRID["e_guess"], W.GRID["cv"], 2.0)
+    dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
+    plan = tdse.BatchPlan(geo, dev)
+    out = plan.replay()
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(out).all())
+    assert float(out[:, 1, :].max()) <= 1.0 + 1e-12
+    pick = [0, 1, 2047, 4095]
+    alone = tdse.propagate_batch(geo, dev[pick[-1:]].contiguous())
+    few = tdse.propagate_batch(geo, dev[pick].contiguous())
+    torch.cuda.synchronize()
+    torch.testing.assert_close(alone[0], out[pick[-1]], rtol=0, atol=0)
+    torch.testing.assert_close(few, out[pick], rtol=0, atol=0)
+    plan.close()
