@@ -239,7 +239,8 @@ int ctdse_launch_config_default(int n_inst, const ctdse_geometry *const *geom, c
  * Every run reads the fields and writes the outputs at the device addresses given at
  * creation, so a caller may refill the field buffer between runs.  Runs of one plan must
  * not overlap (they share the queue counter and the work buffer): order them on one
- * stream.  The geometries must outlive the plan.
+ * stream.  The geometries must outlive the plan.  ctdse_batch_plan_destroy waits for the
+ * plan's last run outside a stream capture before it frees the plan's buffers.
  */
 typedef struct ctdse_batch_plan ctdse_batch_plan;
 int ctdse_batch_plan_create(int n_inst, const ctdse_geometry *const *geom, const int *n_steps, const double *field,

@@ -508,8 +508,14 @@ struct ctdse_batch_plan {
     DevMem work, d_geom, d_inst, d_order, d_counter;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;  // recorded after every run outside a capture
     ~ctdse_batch_plan()
     {
+        // the last run may still be in flight: let it finish before its buffers go
+        if (done) {
+            (void)hipEventSynchronize(done);
+            (void)hipEventDestroy(done);
+        }
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
     }
@@ -650,6 +656,17 @@ int make_plan(const std::vector<InstSpec> &spec, const ctdse_launch_config *cfg,
     HIP_CHECK(hipMemcpy(P.d_geom.p, B.gdev.data(), B.gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(P.d_inst.p, B.insts.data(), B.insts.size() * sizeof(TdseInst), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(P.d_order.p, B.order.data(), B.order.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_CHECK(hipEventCreateWithFlags(&P.done, hipEventDisableTiming));
+    return CTDSE_OK;
+}
+
+// Marks the end of a run on `stream` for the plan's destructor (not while the stream is being
+// captured: there the caller's graph owns the work).
+int mark_done(ctdse_batch_plan &P, hipStream_t stream)
+{
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_CHECK(hipStreamIsCapturing(stream, &cap));
+    if (cap == hipStreamCaptureStatusNone && P.done) HIP_CHECK(hipEventRecord(P.done, stream));
     return CTDSE_OK;
 }
 
@@ -908,7 +925,8 @@ int ctdse_batch_plan_enqueue(ctdse_batch_plan *plan, void *stream)
 {
     clear_status();
     if (!plan) return set_status(CTDSE_ERR_ARGUMENT, "null plan");
-    return enqueue_plan(*plan, plan->work.as<double2>(), (hipStream_t)stream);
+    if (enqueue_plan(*plan, plan->work.as<double2>(), (hipStream_t)stream) != CTDSE_OK) return g_status;
+    return mark_done(*plan, (hipStream_t)stream);
 }
 
 int ctdse_batch_plan_graph_launch(ctdse_batch_plan *plan, void *stream)
@@ -934,7 +952,7 @@ int ctdse_batch_plan_graph_launch(ctdse_batch_plan *plan, void *stream)
         HIP_CHECK(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
     }
     HIP_CHECK(hipGraphLaunch(plan->exec, (hipStream_t)stream));
-    return CTDSE_OK;
+    return mark_done(*plan, (hipStream_t)stream);
 }
 
 int ctdse_batch_plan_info(const ctdse_batch_plan *plan, ctdse_launch_config *config, int *workgroups)

@@ -223,3 +223,34 @@ def test_two_step_passes_with_saves_partial_runs_and_snapshots(monkeypatch, dll)
 
     a, b = _single_vs_pair(monkeypatch, snaps)
     np.testing.assert_array_equal(a, b)
+
+
+def test_plan_enqueue_captured_by_torch_graph():
+    """The plan's enqueue (no allocation, copy or wait) recorded into a torch.cuda.CUDAGraph:
+    every replay, with eager work and a refilled field buffer in between, equals the eager
+    result for the fields it read; destroying the plan after an unsynchronised run is safe."""
+    t, fields = W.gaussian_beam_batch(n_r=3, n_z=2, n_cycles=2)
+    geo = _grid_geometry(2400)
+    dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
+    alt = torch.tensor(tdse.pad_fields(0.5 * fields), device="cuda")
+    ref = tdse.propagate_batch(geo, dev).clone()
+    ref_alt = tdse.propagate_batch(geo, alt).clone()
+    buf = dev.clone()
+    plan = tdse.BatchPlan(geo, buf)
+    plan.enqueue()  # warm-up outside the capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.enqueue()
+    for rep in range(4):
+        buf.copy_(alt if rep % 2 else dev)
+        plan.out.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        junk = (torch.arange(1000, device="cuda", dtype=torch.float64) ** 2).sum()
+        torch.cuda.synchronize()
+        assert float(junk) > 0
+        torch.testing.assert_close(plan.out, ref_alt if rep % 2 else ref, rtol=0, atol=0)
+    del g
+    plan.replay()
+    plan.close()  # waits for the replay still in flight
