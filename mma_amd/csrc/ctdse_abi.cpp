@@ -1215,7 +1215,10 @@ void call1DTDSE(inputs_def *inputs, outputs_def *outputs)
         s.snap_every = spd;
         s.n_snap = n_snap;
     }
-    st = propagate({s}, 0);
+    // one instance alone: the widest segments (8 waves) leave the fewest serial segments per
+    // step (measured 145 ms against 178 ms with 4 waves for 4416 steps of 16001 points)
+    const ctdse_launch_config single{TDSE_MAX_WAVES, 1, 1};
+    st = propagate({s}, 0, &single);
     if (st != CTDSE_OK) {
         delete G;
         return;
