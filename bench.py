@@ -490,17 +490,17 @@ def main():
 
     single = None
     if not args.no_single:
-        # per-instance form: the same propagation for one instance alone, in the shape the
-        # reference-compatible call1DTDSE uses for it (8-wave segments, the fewest per step)
-        one = dev[:1].contiguous()
-        p1 = tdse.BatchPlan(geo, one, config=(8, 1, 1))
-        p1.replay(stream)
+        # per-instance form: the same propagation for one instance alone, as the
+        # reference-compatible call1DTDSE runs it (ctdse_propagate_one: the segments of a step
+        # side by side, one launch per step)
+        one = dev[0].contiguous()
+        o1 = torch.empty((5, n_steps), dtype=torch.float64, device="cuda")
+        tdse.propagate_one(geo, one, n_steps, o1, stream)
         torch.cuda.synchronize()
         s0 = time.perf_counter()
-        p1.replay(stream)
+        tdse.propagate_one(geo, one, n_steps, o1, stream)
         torch.cuda.synchronize()
         single = nx * n_steps / (time.perf_counter() - s0)
-        p1.close()
 
     uneven = None
     if not args.no_uneven:

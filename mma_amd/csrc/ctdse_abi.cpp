@@ -638,6 +638,62 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream, const ctdse
     return CTDSE_OK;
 }
 
+// Waves per workgroup of the step-parallel path (MMA_TDSE_PAR_WAVES overrides): segments of
+// nw*64 - 2H own chunks, one workgroup each, one launch per step.
+int par_waves(int halo_lanes)
+{
+    int nw = 4;
+    if (const char *env = std::getenv("MMA_TDSE_PAR_WAVES")) nw = std::atoi(env);
+    nw = std::max(1, std::min(TDSE_MAX_WAVES, nw));
+    while (nw < TDSE_MAX_WAVES && nw * 64 - 2 * halo_lanes < 64) nw++;
+    return nw * 64 - 2 * halo_lanes >= 64 ? nw : 0;
+}
+
+// One instance over its whole run with no snapshots, final state, partial runs or saved
+// states: the segments of a step are solved side by side (tdse_parallel_step_kernel, psi
+// double-buffered), so a step costs one segment's latency instead of nseg of them.  Returns
+// CTDSE_ERR_UNSUPPORTED (status untouched) when the descriptor needs the persistent path.
+int propagate_steps(const InstSpec &s, hipStream_t stream)
+{
+    if (!s.geom || s.snap || s.psi_final || s.n_run != 0 || s.k0 != 0 || s.psi_init || s.save || s.pregated ||
+        !s.out || s.n_steps < 1)
+        return CTDSE_ERR_UNSUPPORTED;
+    const TdseGeom &g = s.geom->dev;
+    const int nw = par_waves(g.halo_lanes);
+    if (nw == 0) return CTDSE_ERR_UNSUPPORTED;
+    const int own = nw * 64 - 2 * g.halo_lanes;
+    const int nseg = (g.nchunks + own - 1) / own;
+    BatchHost B;
+    if (describe({s}, B) != CTDSE_OK) return g_status;
+    const int n_field = s.n_steps + 1;
+    DevMem states, partial, gated, d_geom, d_inst;
+    HIP_CHECK(states.alloc(2 * (size_t)g.npad * sizeof(double2)));
+    HIP_CHECK(partial.alloc(2 * (size_t)nseg * 5 * sizeof(double)));
+    HIP_CHECK(gated.alloc((size_t)n_field * sizeof(double)));
+    HIP_CHECK(d_geom.alloc(sizeof(TdseGeom)));
+    HIP_CHECK(d_inst.alloc(sizeof(TdseInst)));
+    B.insts[0].field = gated.as<double>();
+    B.insts[0].pregated = 1;
+    double2 *psi[2] = {states.as<double2>(), states.as<double2>() + g.npad};
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    HIP_CHECK(hipMemcpyAsync(d_geom.p, &g, sizeof(TdseGeom), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_inst.p, B.insts.data(), sizeof(TdseInst), hipMemcpyHostToDevice, stream));
+    // psi0 into the first buffer; the second's zero tile and padding rows stay zero (the
+    // steps write own chunks only)
+    HIP_CHECK(hipMemcpyAsync(psi[0], g.psi0, (size_t)g.npad * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+    HIP_CHECK(hipMemsetAsync(psi[1], 0, (size_t)g.npad * sizeof(double2), stream));
+    hipLaunchKernelGGL(tdse_gate_kernel, dim3(1), dim3(256), 0, stream, s.field, n_field, gated.as<double>());
+    for (int k = 0; k < s.n_steps; k++)
+        hipLaunchKernelGGL(tdse_parallel_step_kernel, dim3(nseg, 1), dim3(64 * nw), 0, stream,
+                           d_geom.as<TdseGeom>(), d_inst.as<TdseInst>(), k, psi[k & 1], psi[(k + 1) & 1],
+                           (long long)g.npad, partial.as<double>(), nseg);
+    hipLaunchKernelGGL(tdse_parallel_finish_kernel, dim3(1), dim3(64), 0, stream, d_inst.as<TdseInst>(), 1,
+                       partial.as<double>(), nseg);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return CTDSE_OK;
+}
+
 // Plan path: everything allocated and uploaded here, once.
 int make_plan(const std::vector<InstSpec> &spec, const ctdse_launch_config *cfg, ctdse_batch_plan &P)
 {
@@ -855,6 +911,22 @@ int ctdse_propagate_batch(int n_inst, const ctdse_geometry *const *geom, const i
         spec[i].out = out + out_off[i];
     }
     return propagate(spec, (hipStream_t)stream);
+}
+
+int ctdse_propagate_one(const ctdse_geometry *geom, int n_steps, const double *field, double *out, void *stream)
+{
+    clear_status();
+    if (!geom || !field || !out || n_steps < 1) return set_status(CTDSE_ERR_ARGUMENT, "null or empty argument");
+    if (device_ready() != CTDSE_OK) return g_status;
+    InstSpec s;
+    s.geom = geom;
+    s.n_steps = n_steps;
+    s.field = field;
+    s.out = out;
+    const int st = propagate_steps(s, (hipStream_t)stream);
+    if (st != CTDSE_ERR_UNSUPPORTED) return st;
+    const ctdse_launch_config single{TDSE_MAX_WAVES, 1, 1};
+    return propagate({s}, (hipStream_t)stream, &single);
 }
 
 int ctdse_geometry_create_batch(int n, const int *num_r, const double *dx, const double *dt, const double *a,
@@ -1215,10 +1287,13 @@ void call1DTDSE(inputs_def *inputs, outputs_def *outputs)
         s.snap_every = spd;
         s.n_snap = n_snap;
     }
-    // one instance alone: the widest segments (8 waves) leave the fewest serial segments per
-    // step (measured 145 ms against 178 ms with 4 waves for 4416 steps of 16001 points)
+    // one instance alone: without snapshots its segments run side by side, one launch per
+    // step; with them, the persistent kernel with the widest segments (8 waves: the fewest
+    // serial segments per step, 145 ms against 178 ms with 4 waves for 4416 steps of 16001
+    // points)
     const ctdse_launch_config single{TDSE_MAX_WAVES, 1, 1};
-    st = propagate({s}, 0, &single);
+    st = std::getenv("MMA_TDSE_NO_STEP_PARALLEL") ? (int)CTDSE_ERR_UNSUPPORTED : propagate_steps(s, 0);
+    if (st == CTDSE_ERR_UNSUPPORTED) st = propagate({s}, 0, &single);
     if (st != CTDSE_OK) {
         delete G;
         return;

@@ -73,10 +73,8 @@ struct SegTables {
     double gL, gR;   // g left of the chunk (lane 0) and right of it (lane 63)
 };
 
-__device__ inline void load_tables(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
-                                   SegTables &T)
+__device__ inline void load_tables(const TdseGeom &G, int cs, int wave, int lane, int nw, SegTables &T)
 {
-    const int cs = seg * own_chunks;
     const int nwl = min(nw, (G.nchunks - cs + 63) >> 6);
     const int gl = wave * 64 + lane;
     const int c = cs + gl;
@@ -108,7 +106,10 @@ struct PairLds {
 //            wave-edge points published in LDS (first step of a pair);
 //   kFromRegs: psi_{k+1} from R (own chunks) and LDS (halo chunks, wave-edge neighbours),
 //            psi_{k+2} stored to HBM (second step of a pair).
-enum SegMode { kSingle = 0, kKeep = 1, kFromRegs = 2 };
+//   kParallel: one step of one segment whose window carries a halo on both sides (both
+//            recurrences truncated), so that all segments of a step are independent; psi_k
+//            from `psi`, psi_{k+1} of the own chunks to `psi_out`.
+enum SegMode { kSingle = 0, kKeep = 1, kFromRegs = 2, kParallel = 3 };
 
 // Solve one segment of one step with all waves of the workgroup.
 // observe: add the observables of the state the step starts from (psi_k, read for the
@@ -122,10 +123,11 @@ template <int MODE>
 __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
                               double2 *__restrict__ psi, double2 e0, double2 rot, bool observe, Acc &acc,
                               double2 (*slot_f)[2], double2 (*slot_b)[2], const double2 *edge_in, double2 *edge_out,
-                              double2 (&R)[C], PairLds *pl)
+                              double2 (&R)[C], PairLds *pl, double2 *__restrict__ psi_out = nullptr)
 {
     const int gl = wave * 64 + lane;                 // lane index in the workgroup
-    const int cs = seg * own_chunks;                 // first own chunk
+    // first chunk of the window: the first own chunk, or H chunks before it (kParallel)
+    const int cs = MODE == kParallel ? seg * own_chunks - G.halo_lanes : seg * own_chunks;
     // live waves: those with a lane on the grid (chunks past the grid are zero padding and
     // carry the zero map, so dropping whole waves of them changes nothing)
     const int nwl = min(nw, (G.nchunks - cs + 63) >> 6);
@@ -143,14 +145,15 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     const int par = seg & 1;
 
     SegTables T;
-    load_tables(G, seg, own_chunks, wave, lane, nw, T);
+    load_tables(G, cs, wave, lane, nw, T);
     double2 P[C];
     const double2 zero2 = make_double2(0.0, 0.0);
     double2 nb = zero2;  // lane 0: the point left of the wave's first chunk; lane 63: right of its last
     if constexpr (MODE != kFromRegs) {
 #pragma unroll
         for (int i = 0; i < C; i++) P[i] = ld2(psi, base + 64 * i);
-        if (lane == 0 && gl > 0) nb = ld2(psi, tdse_chunk_base(c - 1) + 64 * (C - 1));  // not yet overwritten
+        if (lane == 0 && (gl > 0 || MODE == kParallel))
+            nb = ld2(psi, tdse_chunk_base(c - 1) + 64 * (C - 1));  // not yet overwritten (chunk -1..-41: zero)
         if (lane == 63 && gl < nl - 1) nb = ld2(psi, tdse_chunk_base(min(c + 1, G.nchunks)));
     } else {
         // own chunks: the previous stage's registers; halo chunks: its successor's first H
@@ -168,7 +171,9 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
             nb = q < own_chunks ? pl->bnd[par][wave + 1][0] : pl->halo[par ^ 1][q - own_chunks][0];
         }
     }
-    const bool own = gl < own_chunks && c < G.nchunks;
+    const bool own = MODE == kParallel
+                         ? gl >= G.halo_lanes && gl < G.halo_lanes + own_chunks && c >= 0 && c < G.nchunks
+                         : gl < own_chunks && c < G.nchunks;
     const int j0 = c * C;
     // observables of the state this step starts from (psi_k)
     if (observe) observe_chunk(G, P, T.dp, base, j0, own, acc);
@@ -176,7 +181,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     const double(&Gv)[C] = T.Gv;
     const double2 Qc = T.Qc, Gc = T.Gc;
     double gL = T.gL, gR = T.gR;
-    if (gl == own_chunks - 1) edge_out[0] = P[C - 1];  // the next segment's left neighbour
+    if (MODE != kParallel && gl == own_chunks - 1) edge_out[0] = P[C - 1];  // the next segment's left neighbour
     // neighbours across the chunk edges: shuffles inside a wave, loads across waves, LDS for
     // the point left of the window, zero past the window's right end
     {
@@ -185,9 +190,11 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
         gR = lane == 63 ? gR : gRd;
     }
     double2 left = dpp2<kWaveShr1>(P[C - 1], zero2), right = dpp2<kWaveShl1>(P[0], zero2);
-    if (lane == 0) left = gl > 0 ? nb : (seg > 0 ? edge_in[0] : zero2);
+    if (lane == 0) left = (gl > 0 || MODE == kParallel) ? nb : (seg > 0 ? edge_in[0] : zero2);
     if (lane == 63) right = nb;
-    const double2 z_in = seg > 0 ? edge_in[1] : zero2;
+    // the forward recurrence enters exactly from the previous segment, or from zero H chunks
+    // before the own chunks (kParallel: truncated like the backward one)
+    const double2 z_in = (MODE != kParallel && seg > 0) ? edge_in[1] : zero2;
 
     // r' = (M2' - i H') psi
     {
@@ -231,7 +238,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
             z = cmul(make_double2(tr, ti), IB[i]);
             P[i] = z;
         }
-        if (gl == own_chunks - 1) edge_out[1] = z;  // the next segment's forward carry
+        if (MODE != kParallel && gl == own_chunks - 1) edge_out[1] = z;  // the next segment's forward carry
     }
 
     // backward recurrence u_j = z_j + h_j u_{j+1},  h_j = -(1 + i g_{j+1}) ib_j
@@ -273,9 +280,10 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
         if (lane == 0) pl->bnd[par][wave][0] = R[0];
         if (lane == 63) pl->bnd[par][wave][1] = R[C - 1];
     } else {
+        double2 *dst = MODE == kParallel ? psi_out : psi;
 #pragma unroll
         for (int i = 0; i < C; i++) {
-            st2(psi, base + 64 * i, cmul(e, P[i]));
+            st2(dst, base + 64 * i, cmul(e, P[i]));
             e = cmul(e, rot);
         }
     }
@@ -532,6 +540,25 @@ __device__ void propagate_loop(const TdseGeom *__restrict__ geoms, const TdseIns
     }
 }
 
+// Entry idx of the outputs from the per-segment partial sums of a step-parallel run, added in
+// segment order; F is the gated field of the step that produced the state.  One thread.
+__device__ inline void store_partials(const TdseInst &I, const double *parts, int nseg, int idx, double F)
+{
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+    for (int q = 0; q < nseg; q++) {
+        s0 += parts[5 * q + 0];
+        s1 += parts[5 * q + 1];
+        s2 += parts[5 * q + 2];
+        s3 += parts[5 * q + 3];
+        s4 += parts[5 * q + 4];
+    }
+    I.src[idx] = -s4 + F;
+    I.ptot[idx] = s0 * s0 + s1 * s1;
+    I.pint[idx] = s2;
+    I.expv[idx] = s3;
+    if (I.efield) I.efield[idx] = F;
+}
+
 } // namespace
 
 // One time step per pass over psi (168-VGPR budget, 12 waves per CU).
@@ -550,4 +577,89 @@ tdse_propagate_pair_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *_
 {
     __shared__ PairLds pl;
     propagate_loop<true>(geoms, insts, order, n_inst, counter, work, work_stride, &pl);
+}
+
+// One time step k of instance blockIdx.y, split over independent segments (blockIdx.x) whose
+// windows carry a halo on both sides: psi_k from psi_in, psi_{k+1} to psi_out (the caller
+// alternates the two).  The field is the gated one (I.field[k] = F_k).  The observables of
+// psi_k (k >= 1) go to partial[k & 1][instance][segment]; the next launch sums those of
+// entry k - 1 in segment order (tdse_parallel_finish_kernel the last).  For one instance
+// alone this runs a step in the time of one segment instead of all of them in turn.
+__global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_WAVES_PER_SIMD)
+tdse_parallel_step_kernel(const TdseGeom *__restrict__ geom, const TdseInst *__restrict__ insts, int k,
+                          const double2 *psi_in, double2 *psi_out, long long state_stride, double *partial, int nseg)
+{
+    __shared__ double part[TDSE_MAX_WAVES][5];
+    __shared__ double2 slot_f[TDSE_MAX_WAVES][2], slot_b[TDSE_MAX_WAVES][2];
+    const int nw = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int inst = blockIdx.y, seg = blockIdx.x, n_inst = gridDim.y;
+    const TdseInst &I = insts[inst];
+    const TdseGeom &G = *geom;
+    const int own_chunks = nw * 64 - 2 * G.halo_lanes;  // host guarantees >= 64
+    if (seg == 0 && threadIdx.x == 0 && I.src) {
+        if (k == 0) {
+            I.src[0] = 0.0;
+            I.ptot[0] = 1.0;
+            I.pint[0] = G.pint0;
+            I.expv[0] = 0.0;
+            if (I.efield) I.efield[0] = 0.0;
+        } else if (k >= 2) {
+            store_partials(I, partial + ((long long)((k - 1) & 1) * n_inst + inst) * nseg * 5, nseg, k - 1,
+                           I.field[k - 2]);
+        }
+    }
+    const double F = I.field[k];
+    const double Fdt = F * G.dt;
+    double2 rot, e0;
+    sincos(Fdt * G.dx, &rot.y, &rot.x);
+    const int c = seg * own_chunks - G.halo_lanes + wave * 64 + lane;  // this lane's chunk
+    sincos(Fdt * __dsub_rn(__dmul_rn((double)(c * C), G.dx), G.xmax), &e0.y, &e0.x);
+    Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double2 R[C];  // unused by this mode
+    solve_segment<kParallel>(G, seg, own_chunks, wave, lane, nw,
+                             const_cast<double2 *>(psi_in) + (long long)inst * state_stride, e0, rot, k >= 1, acc,
+                             slot_f, slot_b, nullptr, nullptr, R, nullptr,
+                             psi_out + (long long)inst * state_stride);
+    wave_observables(acc, wave, lane, part);
+    __syncthreads();
+    if (threadIdx.x == 0 && k >= 1) {
+        double *dst = partial + (((long long)(k & 1) * n_inst + inst) * nseg + seg) * 5;
+        for (int q = 0; q < 5; q++) {
+            double sum = 0.0;
+            for (int w = 0; w < nw; w++) sum += part[w][q];
+            dst[q] = sum;
+        }
+    }
+}
+
+// The last observed entry (n_steps - 1) of a step-parallel run: one thread per instance.
+__global__ void tdse_parallel_finish_kernel(const TdseInst *insts, int n_inst, const double *partial, int nseg)
+{
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= n_inst) return;
+    const TdseInst &I = insts[inst];
+    const int last = I.n_steps - 1;
+    if (!I.src || last < 1) return;
+    store_partials(I, partial + ((long long)(last & 1) * n_inst + inst) * nseg * 5, nseg, last,
+                   last >= 1 ? I.field[last - 1] : 0.0);
+}
+
+// The gated field of prop.c:125-134: F_k = 0 up to and including the first k with
+// f_k f_{k+1} <= 0, f_k after it; n samples (the last is read only as f_{k+1}).  One block.
+__global__ void tdse_gate_kernel(const double *f, int n, double *F)
+{
+    __shared__ int kg;
+    if (threadIdx.x == 0) {
+        int g = n;
+        for (int k = 0; k + 1 < n; k++)
+            if (f[k] * f[k + 1] <= 0.0) {
+                g = k;
+                break;
+            }
+        kg = g;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) F[k] = k <= kg ? 0.0 : f[k];
 }

@@ -51,6 +51,8 @@ def _lib():
         L.ctdse_propagate_batch.restype = c_int
         L.ctdse_propagate_batch.argtypes = [c_int, POINTER(c_void_p), POINTER(c_int), c_void_p, POINTER(c_longlong),
                                             c_void_p, POINTER(c_longlong), c_void_p]
+        L.ctdse_propagate_one.restype = c_int
+        L.ctdse_propagate_one.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p]
         L.ctdse_geometry_create_batch.restype = c_int
         L.ctdse_geometry_create_batch.argtypes = [c_int, POINTER(c_int)] + [POINTER(c_double)] * 6 + [
             POINTER(c_void_p), POINTER(c_int)]
@@ -355,6 +357,28 @@ def propagate_stepwise(geometry, fields, out=None, stream=None):
     return out
 
 
+def propagate_one(geometry, field, n_steps=None, out=None, stream=None):
+    """One instance alone (ctdse_propagate_one): the propagation call1DTDSE runs, with the
+    segments of each step side by side.  field: device float64 samples (at least n_steps + 1;
+    default n_steps = len(field) - 1); returns out, (5, n_steps) rows as propagate_batch's.
+    Equal to propagate_batch's row to rounding, not bitwise."""
+    field = field.reshape(-1)
+    _check_device(field, "field")
+    n_steps = field.numel() - 1 if n_steps is None else int(n_steps)
+    if n_steps < 1 or field.numel() < n_steps + 1:
+        raise ValueError("field needs n_steps + 1 samples")
+    if out is None:
+        out = torch.empty((5, n_steps), dtype=torch.float64, device=field.device)
+    _check_device(out, "out")
+    if out.numel() < 5 * n_steps:
+        raise ValueError("out needs 5 * n_steps doubles")
+    if stream is None:
+        stream = torch.cuda.current_stream(field.device)
+    _raise(_lib().ctdse_propagate_one(geometry.handle, n_steps, c_void_p(field.data_ptr()), c_void_p(out.data_ptr()),
+                                      c_void_p(stream.cuda_stream)))
+    return out
+
+
 def _launch(geoms, n_steps, fields, field_off, out, out_off, stream, result):
     n_inst = len(n_steps)
     handles = (c_void_p * n_inst)(*[g.handle.value for g in geoms])
@@ -377,5 +401,5 @@ def pad_fields(fields_np, extra=3):
 
 
 __all__ = ["Geometry", "BatchPlan", "LaunchConfig", "default_config", "propagate_batch", "propagate_ragged",
-           "propagate_stepwise",
+           "propagate_stepwise", "propagate_one",
            "pad_fields", "OUTPUT_ROWS", "STATUS_NAMES"]

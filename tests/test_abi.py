@@ -95,6 +95,7 @@ def test_batched_entry_points_reject_bad_arguments_before_touching_a_device():
     assert lib.ctdse_adjoint_batch(1, N, N, N, N, N, N, N, N, N, N, N, N, N, N) == 3
     assert lib.ctdse_propagate_states(1, N, N, N, N, N, N, N, N, N, N, 1, N, N, N) == 3
     assert lib.ctdse_propagate_batch(1, N, N, N, N, N, N, N) == 3
+    assert lib.ctdse_propagate_one(N, 10, N, N, N) == 3
     assert lib.ctdse_adjoint_reverse_batch(0, N, N, N, N, N, N, N, N, N, N, 1, 0, N, N, N) in (0, 5)  # empty batch
 
 

@@ -102,3 +102,25 @@ def test_full_gaussian_beam_batch_invariants():
     torch.testing.assert_close(alone[0], out[pick[-1]], rtol=0, atol=0)
     torch.testing.assert_close(few, out[pick], rtol=0, atol=0)
     plan.close()
+
+
+def test_propagate_one_matches_batch_rows():
+    """ctdse_propagate_one (segments of a step side by side, a halo on both sides of each)
+    against the batch rows of the same instances, on grids of one segment up to many and
+    with steps fewer than the field holds: equal to rounding; the Efield row bitwise."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    for num_r, dx in ((300, 0.2), (2000, 0.1), (16000, 0.1), (9001, 0.05)):
+        geo = tdse.Geometry(num_r, dx, 0.25, W.GRID["a"], W.GRID["e_guess"], 1e-25, 2.0)
+        tt = np.arange(700) * 0.25
+        f = W.sin2_pulse(tt, 0.08, W.omega_au(800e-9), 3, phase=rng.uniform(0, 3))
+        dev = torch.tensor(tdse.pad_fields(f), device="cuda")
+        for n_steps in (dev.shape[1] - 1, 401):
+            ref = tdse.propagate_batch(geo, dev, n_steps=[n_steps])[0]
+            got = tdse.propagate_one(geo, dev[0].contiguous(), n_steps)
+            torch.cuda.synchronize()
+            got, ref = got.cpu().numpy(), ref.cpu().numpy()
+            np.testing.assert_array_equal(got[4], ref[4])
+            for r, k in enumerate(ROWS[:4]):
+                e = rel_err(got[r], ref[r])
+                print(num_r, n_steps, k, "%.2e" % e, flush=True)
+                assert e < 1e-12, (num_r, n_steps, k)

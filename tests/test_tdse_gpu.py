@@ -77,6 +77,28 @@ def test_single_instance_800nm(dll):
     inputs.delete(dll)
 
 
+def test_single_instance_paths_agree(dll, monkeypatch):
+    """call1DTDSE without snapshots runs the segments of each step side by side (both
+    recurrences truncated at a halo on each side); the persistent kernel (forward recurrence
+    exact across segments) gives the same series to rounding, and both match the reference."""
+    g = golden("ref_single.npz")
+    inputs, par = run_case(dll, g)
+    monkeypatch.setenv("MMA_TDSE_NO_STEP_PARALLEL", "1")
+    inputs2, seq = run_case(dll, g)
+    monkeypatch.delenv("MMA_TDSE_NO_STEP_PARALLEL")
+    np.testing.assert_array_equal(par.get_Efield(), seq.get_Efield())
+    for k in SERIES:
+        a, b = getattr(par, "get_" + k)(), getattr(seq, "get_" + k)()
+        e = rel_err(a, b)
+        print("%-10s parallel vs persistent %.3e" % (k, e), flush=True)
+        assert e < 1e-12, k
+    check_series(par, g)
+    for o in (par, seq):
+        o.delete(dll)
+    for i in (inputs, inputs2):
+        i.delete(dll)
+
+
 def test_reference_unit_test_case(dll):
     """1DTDSE/tests.py:10-88: same inputs, compared with the golden vector (1e-8) and with the
     reference's fixture ionization.h5 (np.allclose, as the reference test does)."""
