@@ -51,6 +51,8 @@ HkConst make_constants()
 }
 
 const double kInvC = 1.0 / 299792458.0;
+// largest q grid: its bins (16 B each) and the spread kernel's few static bytes fit the 160 KiB of LDS
+const int kMaxBins = 10176;
 
 int check_planes(const hk_planes *p)
 {
@@ -185,7 +187,9 @@ int hk_grid_size(double q_max, double rho_max, double tol, int *nq, double *h)
     const double step = std::pow(tol / (nodal / fact), 1.0 / P);  // allowed rho_max * h
     const double need = rho_max * q_max / step + P + 1;
     const long long n = ((long long)std::ceil(need) + 63) / 64 * 64;
-    if (n > 10240) return fail(HK_ERR_UNSUPPORTED, "q grid of " + std::to_string(n) + " points exceeds LDS (10240)");
+    if (n > kMaxBins)
+        return fail(HK_ERR_UNSUPPORTED,
+                    "q grid of " + std::to_string(n) + " points exceeds LDS (" + std::to_string(kMaxBins) + ")");
     *nq = (int)n;
     *h = q_max * (1.0 + 1e-12) / (double)(n - P - 1);
     return HK_OK;
@@ -196,7 +200,7 @@ int hk_grid_spread(const hk_planes *p, const double *weights, int nq, double h, 
 {
     clear_status();
     if (check_planes(p) != HK_OK) return g_status;
-    if (!weights || !out || nq < HK_GRID_P + 2 || nq > 10240 || ld < nq || row0 < 0 || !(h > 0))
+    if (!weights || !out || nq < HK_GRID_P + 2 || nq > kMaxBins || ld < nq || row0 < 0 || !(h > 0))
         return fail(HK_ERR_ARGUMENT, "bad spread arguments");
     if (p->pre_mode == 2) return fail(HK_ERR_UNSUPPORTED, "the gridded contraction needs an r-independent pre-factor");
     HostGrids g;
@@ -209,12 +213,12 @@ int hk_grid_spread(const hk_planes *p, const double *weights, int nq, double h, 
     if (type == HK_F64) {
         HK_HIP(hipFuncSetAttribute((const void *)hk_spread_kernel<HK_F64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
-        hipLaunchKernelGGL(hk_spread_kernel<HK_F64>, dim3(p->no), dim3(256), lds, s, *p, weights, nq, h, kInvC, out,
+        hipLaunchKernelGGL(hk_spread_kernel<HK_F64>, dim3(p->no), dim3(1024), lds, s, *p, weights, nq, h, kInvC, out,
                            ld, row0);
     } else if (type == HK_F32) {
         HK_HIP(hipFuncSetAttribute((const void *)hk_spread_kernel<HK_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
-        hipLaunchKernelGGL(hk_spread_kernel<HK_F32>, dim3(p->no), dim3(256), lds, s, *p, weights, nq, h, kInvC, out,
+        hipLaunchKernelGGL(hk_spread_kernel<HK_F32>, dim3(p->no), dim3(1024), lds, s, *p, weights, nq, h, kInvC, out,
                            ld, row0);
     } else {
         return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");

@@ -40,6 +40,7 @@ __device__ inline double2 source_term(const hk_planes &p, const S *plane, int k,
 }
 
 constexpr int kThreads = 256;
+constexpr int kSpreadThreads = 1024;  // hk_spread_kernel's workgroup
 constexpr int kJTile = 512;     // radial points staged in LDS per pass
 constexpr int kAnchor = 32;     // exact sincos every kAnchor rotated steps
 
@@ -148,15 +149,15 @@ struct Lagrange {
         suf[P - 1] = 1.0;
 #pragma unroll
         for (int i = P - 2; i >= 0; i--) suf[i] = suf[i + 1] * t[i + 1];
-        // 1 / prod_{j != i} (d_i - d_j) = (-1)^(P-1-i) / (i! (P-1-i)!)
-        double fact[P];
-        fact[0] = 1.0;
-#pragma unroll
-        for (int i = 1; i < P; i++) fact[i] = fact[i - 1] * (double)i;
+        // 1 / prod_{j != i} (d_i - d_j) = (-1)^(P-1-i) / (i! (P-1-i)!), as a product with
+        // the correctly rounded reciprocal (a division per weight costs more than the rest)
 #pragma unroll
         for (int i = 0; i < P; i++) {
-            const double den = fact[i] * fact[P - 1 - i];
-            L[i] = ((P - 1 - i) & 1 ? -1.0 : 1.0) * (pre[i] * suf[i]) / den;
+            double den = 1.0;
+            for (int q = 2; q <= i; q++) den *= (double)q;
+            for (int q = 2; q <= P - 1 - i; q++) den *= (double)q;
+            const double inv = ((P - 1 - i) & 1 ? -1.0 : 1.0) / den;  // folded at compile time
+            L[i] = (pre[i] * suf[i]) * inv;
         }
     }
 };
@@ -172,50 +173,70 @@ __device__ inline void store_operand(void *out, long long at, double v)
 
 } // namespace
 
-// one workgroup per frequency; the nq bins of C(., omega) accumulate in LDS (FP64 atomics)
+// one workgroup per frequency; the nq bins of C(., omega) accumulate in LDS (FP64 atomics,
+// real and imaginary parts in separate arrays: fewer bank conflicts than interleaved pairs).
+// The (plane, radius) pairs of the planes with non-zero weight are dealt to the 1024 threads
+// in one flat range, advanced with 32-bit counters, so every wave has work whatever nr is.
 template <int TYPE>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kSpreadThreads)
 hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double inv_c, void *out, long long ld,
                  long long row0)
 {
-    extern __shared__ double2 acc[];
+    extern __shared__ double bins[];
+    __shared__ int k_range[2];
+    double *acc_re = bins, *acc_im = bins + nq;
     const int o = blockIdx.x;
-    for (int l = threadIdx.x; l < nq; l += kThreads) acc[l] = make_double2(0.0, 0.0);
+    for (int l = threadIdx.x; l < 2 * nq; l += blockDim.x) bins[l] = 0.0;
+    if (threadIdx.x == 0) {
+        k_range[0] = p.nz;
+        k_range[1] = -1;
+    }
     __syncthreads();
+    for (int k = threadIdx.x; k < p.nz; k += blockDim.x)
+        if (weights[k] != 0.0) {
+            atomicMin(&k_range[0], k);
+            atomicMax(&k_range[1], k);
+        }
+    __syncthreads();
+    const int k_end = k_range[1] + 1;
     const double K = p.ogrid[o] * inv_c;
     const double inv_h = 1.0 / h;
     const bool c64 = p.src_c64 != 0;
-    for (int k = 0; k < p.nz; k++) {
+    const int step_k = blockDim.x / p.nr, step_j = blockDim.x - step_k * p.nr;
+    int k = k_range[0] + threadIdx.x / p.nr, j = threadIdx.x % p.nr;
+    for (; k < k_end; k += step_k, j += step_j) {
+        if (j >= p.nr) {  // step_j < nr: one wrap at most
+            j -= p.nr;
+            if (++k >= k_end) break;
+        }
         const double wk = weights[k];
         if (wk == 0.0) continue;
         const double D = p.distance - p.zgrid[k];
         const double s = K / D;
         double2 pk = make_double2(1.0, 0.0);
         if (p.pre_mode == 1) pk = reinterpret_cast<const double2 *>(p.pre)[(long long)k * p.no + o];
-        for (int j = threadIdx.x; j < p.nr; j += kThreads) {
-            double2 v = c64 ? source_term(p, reinterpret_cast<const float2 *>(p.src) + (long long)k * p.plane_stride,
-                                          k, o, j, K, D, pk)
-                            : source_term(p, reinterpret_cast<const double2 *>(p.src) + (long long)k * p.plane_stride,
-                                          k, o, j, K, D, pk);
-            v.x *= wk;
-            v.y *= wk;
-            const double t = s * p.rgrid[j] * inv_h;
-            const double b = floor(t);
-            const Lagrange lw(t - b);
-            const int l0 = (int)b + 1;  // bin of node offset -(P/2 - 1): Q_l = (l - P/2) h
-            if (l0 < 0 || l0 + HK_GRID_P > nq) continue;  // the host sizes the grid so this never holds
+        double2 v = c64 ? source_term(p, reinterpret_cast<const float2 *>(p.src) + (long long)k * p.plane_stride, k,
+                                      o, j, K, D, pk)
+                        : source_term(p, reinterpret_cast<const double2 *>(p.src) + (long long)k * p.plane_stride, k,
+                                      o, j, K, D, pk);
+        v.x *= wk;
+        v.y *= wk;
+        const double t = s * p.rgrid[j] * inv_h;
+        const double b = floor(t);
+        const Lagrange lw(t - b);
+        const int l0 = (int)b + 1;  // bin of node offset -(P/2 - 1): Q_l = (l - P/2) h
+        if (l0 < 0 || l0 + HK_GRID_P > nq) continue;  // the host sizes the grid so this never holds
 #pragma unroll
-            for (int i = 0; i < HK_GRID_P; i++) {
-                atomicAdd(&acc[l0 + i].x, lw.L[i] * v.x);
-                atomicAdd(&acc[l0 + i].y, lw.L[i] * v.y);
-            }
+        for (int i = 0; i < HK_GRID_P; i++) {
+            atomicAdd(&acc_re[l0 + i], lw.L[i] * v.x);
+            atomicAdd(&acc_im[l0 + i], lw.L[i] * v.y);
         }
     }
     __syncthreads();
     const long long re_row = row0 + 2LL * o, im_row = re_row + 1;
-    for (int l = threadIdx.x; l < nq; l += kThreads) {
-        store_operand<TYPE>(out, re_row * ld + l, acc[l].x);
-        store_operand<TYPE>(out, im_row * ld + l, acc[l].y);
+    for (int l = threadIdx.x; l < nq; l += blockDim.x) {
+        store_operand<TYPE>(out, re_row * ld + l, acc_re[l]);
+        store_operand<TYPE>(out, im_row * ld + l, acc_im[l]);
     }
 }
 
