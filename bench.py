@@ -210,6 +210,27 @@ def hankel_cpu_baseline(zg, rg, og, rho, src, pre, distance, threads, target_s=6
                       % (n_tasks, nrho, nr, threads, dt, float(nz) * no * nrho * nr, float(nz) * no * nrho * nr * dt / elems)}
 
 
+def single_measure(geo, field, n_steps, nx, reps=1):
+    """Per-instance form: one instance alone, as the reference-compatible call1DTDSE runs it
+    (ctdse_propagate_one: the segments of a step side by side, one launch per step); best of
+    `reps` timed calls after one untimed call, allocation and the final wait included."""
+    import torch
+    from mma_amd import tdse
+    one = field.contiguous()
+    o1 = torch.empty((5, n_steps), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    tdse.propagate_one(geo, one, n_steps, o1, stream)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        s0 = time.perf_counter()
+        tdse.propagate_one(geo, one, n_steps, o1, stream)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - s0)
+    return {"pt_steps_per_s": nx * n_steps / best, "seconds": best, "Nx": nx, "steps": n_steps,
+            "launches": n_steps, "form": "ctdse_propagate_one"}
+
+
 def hankel_measure(cpu=True):
     """Far-field workload: 256 source planes (5 mm Ar cell, z-modulated pressure, NIST f1 /
     Henke f2) x 2048 frequencies (harmonics 11-61 of 800 nm) x 512 radii (0-200 um) onto 1024
@@ -394,6 +415,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the 8192-instance uneven workload")
     ap.add_argument("--no-stepwise", action="store_true", help="skip the one-launch-per-step form")
     ap.add_argument("--only-uneven", action="store_true", help="time only the uneven workload")
+    ap.add_argument("--only-single", action="store_true", help="time only one instance alone (call1DTDSE's form)")
     ap.add_argument("--launch-config", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
                     metavar="W,G,S", help="launch shape (waves/workgroup, workgroups/CU, steps/pass) instead "
                                           "of the autotuner's (profiling runs: no trial launches)")
@@ -427,6 +449,16 @@ def main():
         return
     if args.only_uneven:
         print(json.dumps({"uneven": uneven_measure(config=args.launch_config)}), flush=True)
+        return
+    if args.only_single:
+        import torch
+        from mma_amd import tdse, workloads as W
+        g = W.GRID
+        geo = tdse.Geometry(g["num_r"], g["dx"], 0.25, g["a"], g["e_guess"], g["cv"], g["x_int"])
+        t, fields = W.gaussian_beam_batch(n_r=1, n_z=1)
+        dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
+        print(json.dumps({"single": single_measure(geo, dev[0], fields.shape[1] + 2, g["num_r"] + 1, reps=3)}),
+              flush=True)
         return
     if args.only_adjoint:
         import torch
@@ -490,17 +522,7 @@ def main():
 
     single = None
     if not args.no_single:
-        # per-instance form: the same propagation for one instance alone, as the
-        # reference-compatible call1DTDSE runs it (ctdse_propagate_one: the segments of a step
-        # side by side, one launch per step)
-        one = dev[0].contiguous()
-        o1 = torch.empty((5, n_steps), dtype=torch.float64, device="cuda")
-        tdse.propagate_one(geo, one, n_steps, o1, stream)
-        torch.cuda.synchronize()
-        s0 = time.perf_counter()
-        tdse.propagate_one(geo, one, n_steps, o1, stream)
-        torch.cuda.synchronize()
-        single = nx * n_steps / (time.perf_counter() - s0)
+        single = single_measure(geo, dev[0], n_steps, nx)["pt_steps_per_s"]
 
     uneven = None
     if not args.no_uneven:
