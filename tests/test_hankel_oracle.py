@@ -128,6 +128,30 @@ def test_z_modulated_constant_pressure_equals_scalar():
     assert np.max(np.abs(ra - rb)) <= 1e-11 * np.max(np.abs(rb))
 
 
+def test_z_and_r_modulated_separable_pressure_matches_z_branch():
+    """The z-and-r branch (Hankel_transform.py:244-399) on a separable table p(z) s(r) given at
+    the r nodes: at each radius it is the z-modulated branch (itself pinned to the scalar one
+    above) for the table s(r) p(z), without that branch's exit-plane renormalisation (:215 has
+    no counterpart there), i.e. times renorm[0] when the first plane sits at the table's start;
+    beta comes from the dispersion tables there (:373-376), so both tables are NIST here.
+    A transposed (z, r) table, a wrong radius or a renormalised exit plane fails."""
+    z = np.linspace(0, 5e-3, 9)
+    r = np.linspace(0, 1e-4, 4)
+    o = OMEGA0 * np.linspace(17, 45, 5)
+    zp = np.linspace(0, 6e-3, 41)
+    pz = 0.5 + 0.4 * np.sin(zp / 6e-3 * 3.0)
+    s = np.array([1.0, 0.8, 0.55, 0.3])
+    table = {"zgrid": zp, "rgrid": r, "value": pz[:, None] * s[None, :]}
+    pre, ren = H.prefactor_tables(z, r, o, "Ar", table, absorption_tables="NIST", dispersion_tables="NIST")
+    assert ren is None and pre.shape == (9, 5, 4)
+    for j in range(4):
+        a, ra = H.prefactor_tables(z, r, o, "Ar", {"zgrid": zp, "value": pz * s[j]}, absorption_tables="NIST",
+                                   dispersion_tables="NIST")
+        want = a * ra[0][None, :]
+        assert np.max(np.abs(pre[:, :, j] - want)) <= 1e-11 * np.max(np.abs(want)), j
+    assert np.max(np.abs(ra[0] - 1.0)) > 1e-3  # the absorption term is not negligible here
+
+
 def test_unlisted_f1_raises():
     with pytest.raises(ValueError):
         H.dispersion_function(OMEGA0 * 7, 1.0, "Ar_Henke")  # 10.8 eV: f1 = -9999 in the Henke table
