@@ -633,9 +633,9 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream, const ctdse
     HIP_CHECK(hipMemcpyAsync(P.d_geom.p, B.gdev.data(), B.gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(P.d_inst.p, B.insts.data(), B.insts.size() * sizeof(TdseInst), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(P.d_order.p, B.order.data(), B.order.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    if (enqueue_plan(P, g_work.as<double2>(), stream) != CTDSE_OK) return g_status;
-    HIP_CHECK(hipStreamSynchronize(stream));
-    return CTDSE_OK;
+    const int queued = enqueue_plan(P, g_work.as<double2>(), stream);
+    HIP_CHECK(hipStreamSynchronize(stream));  // the descriptors are freed on return
+    return queued;
 }
 
 // Waves per workgroup of the step-parallel path (MMA_TDSE_PAR_WAVES overrides): segments of
@@ -689,8 +689,10 @@ int propagate_steps(const InstSpec &s, hipStream_t stream)
                            (long long)g.npad, partial.as<double>(), nseg);
     hipLaunchKernelGGL(tdse_parallel_finish_kernel, dim3(1), dim3(64), 0, stream, d_inst.as<TdseInst>(), 1,
                        partial.as<double>(), nseg);
-    HIP_CHECK(hipGetLastError());
+    // wait before reporting a launch error: the buffers freed on return may still be in use
+    const hipError_t launched = hipGetLastError();
     HIP_CHECK(hipStreamSynchronize(stream));
+    HIP_CHECK(launched);
     return CTDSE_OK;
 }
 
