@@ -198,13 +198,13 @@ int ctdse_propagate_batch(int n_inst, const ctdse_geometry *const *geom, const i
                           const long long *field_off, double *out, const long long *out_off, void *stream);
 
 /*
- * One instance alone, the propagation call1DTDSE runs when no snapshots are asked for (no
- * counterpart of its own in the reference: 1DTDSE/sources/singleTDSE.c:105 calls propagation
- * there).
- * Arguments as for one instance of ctdse_propagate_batch.  The segments of each step run
- * side by side, one launch per step, so one instance gets most of the GPU; both recurrences
- * are truncated at a halo on either side of a segment, so results equal the batch's to
- * rounding (about 1e-15), not bitwise.  Returns once the work queued on `stream` has finished.
+ * One instance alone, the propagation call1DTDSE runs (no counterpart of its own in the
+ * reference: 1DTDSE/sources/singleTDSE.c:105 calls propagation there).  Arguments as for
+ * one instance of ctdse_propagate_batch.  The segments of each step run side by side, one
+ * launch per step, so one instance gets most of the GPU; both recurrences are truncated at a
+ * halo on either side of a segment, so results equal the batch's to rounding (about 1e-13
+ * relative on the dipole), not bitwise.  Returns once the work queued on `stream` has
+ * finished.
  */
 int ctdse_propagate_one(const ctdse_geometry *geom, int n_steps, const double *field, double *out, void *stream);
 

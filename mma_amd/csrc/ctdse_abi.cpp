@@ -649,14 +649,14 @@ int par_waves(int halo_lanes)
     return nw * 64 - 2 * halo_lanes >= 64 ? nw : 0;
 }
 
-// One instance over its whole run with no snapshots, final state, partial runs or saved
-// states: the segments of a step are solved side by side (tdse_parallel_step_kernel, psi
-// double-buffered), so a step costs one segment's latency instead of nseg of them.  Returns
-// CTDSE_ERR_UNSUPPORTED (status untouched) when the descriptor needs the persistent path.
+// One instance over its whole run with no partial runs or saved states: the segments of a
+// step are solved side by side (tdse_parallel_step_kernel, psi double-buffered), so a step
+// costs one segment's latency instead of nseg of them; snapshots and the final state are
+// copied out between steps.  Returns CTDSE_ERR_UNSUPPORTED (status untouched) when the
+// descriptor needs the persistent path.
 int propagate_steps(const InstSpec &s, hipStream_t stream)
 {
-    if (!s.geom || s.snap || s.psi_final || s.n_run != 0 || s.k0 != 0 || s.psi_init || s.save || s.pregated ||
-        !s.out || s.n_steps < 1)
+    if (!s.geom || s.n_run != 0 || s.k0 != 0 || s.psi_init || s.save || s.pregated || !s.out || s.n_steps < 1)
         return CTDSE_ERR_UNSUPPORTED;
     const TdseGeom &g = s.geom->dev;
     const int nw = par_waves(g.halo_lanes);
@@ -683,10 +683,25 @@ int propagate_steps(const InstSpec &s, hipStream_t stream)
     HIP_CHECK(hipMemcpyAsync(psi[0], g.psi0, (size_t)g.npad * sizeof(double2), hipMemcpyDeviceToDevice, stream));
     HIP_CHECK(hipMemsetAsync(psi[1], 0, (size_t)g.npad * sizeof(double2), stream));
     hipLaunchKernelGGL(tdse_gate_kernel, dim3(1), dim3(256), 0, stream, s.field, n_field, gated.as<double>());
-    for (int k = 0; k < s.n_steps; k++)
+    const dim3 untile_grid((g.nx + 255) / 256);
+    const int n_snap = s.snap ? s.n_snap : 0, every = std::max(1, s.snap_every);
+    int i_snap = 0;
+    for (int k = 0; k < s.n_steps; k++) {
         hipLaunchKernelGGL(tdse_parallel_step_kernel, dim3(nseg, 1), dim3(64 * nw), 0, stream,
                            d_geom.as<TdseGeom>(), d_inst.as<TdseInst>(), k, psi[k & 1], psi[(k + 1) & 1],
                            (long long)g.npad, partial.as<double>(), nseg);
+        // prop.c:248-255: row i_snap gets psi after step k when k % every == 0
+        if (k % every == 0 && i_snap < n_snap - 1)
+            hipLaunchKernelGGL(tdse_untile_kernel, untile_grid, dim3(256), 0, stream, psi[(k + 1) & 1], g.nx,
+                               s.snap + (size_t)i_snap++ * 2 * g.nx);
+    }
+    // prop.c:256-262: after the last step, the last row gets the final state if every earlier
+    // row was used; the final state itself is optional output
+    const double2 *fin = psi[s.n_steps & 1];
+    if (n_snap > 0 && i_snap == n_snap - 1)
+        hipLaunchKernelGGL(tdse_untile_kernel, untile_grid, dim3(256), 0, stream, fin, g.nx,
+                           s.snap + (size_t)(n_snap - 1) * 2 * g.nx);
+    if (s.psi_final) hipLaunchKernelGGL(tdse_untile_kernel, untile_grid, dim3(256), 0, stream, fin, g.nx, s.psi_final);
     hipLaunchKernelGGL(tdse_parallel_finish_kernel, dim3(1), dim3(64), 0, stream, d_inst.as<TdseInst>(), 1,
                        partial.as<double>(), nseg);
     // wait before reporting a launch error: the buffers freed on return may still be in use
@@ -1289,8 +1304,8 @@ void call1DTDSE(inputs_def *inputs, outputs_def *outputs)
         s.snap_every = spd;
         s.n_snap = n_snap;
     }
-    // one instance alone: without snapshots its segments run side by side, one launch per
-    // step; with them, the persistent kernel with the widest segments (8 waves: the fewest
+    // one instance alone: its segments run side by side, one launch per step; where that
+    // path cannot run, the persistent kernel with the widest segments (8 waves: the fewest
     // serial segments per step, 145 ms against 178 ms with 4 waves for 4416 steps of 16001
     // points)
     const ctdse_launch_config single{TDSE_MAX_WAVES, 1, 1};

@@ -342,6 +342,16 @@ __global__ void tdse_stream_copy_kernel(const double2 *__restrict__ src, double2
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// A tiled state in natural order, interleaved re, im (a snapshot row of prop.c:248-263)
+__global__ void tdse_untile_kernel(const double2 *psi, int nx, double *row)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nx) return;
+    const double2 v = psi[tdse_index(j)];
+    row[2 * j] = v.x;
+    row[2 * j + 1] = v.y;
+}
+
 __global__ void tdse_fill_int_kernel(int *p, int n, int v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

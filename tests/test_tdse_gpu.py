@@ -78,13 +78,13 @@ def test_single_instance_800nm(dll):
 
 
 def test_single_instance_paths_agree(dll, monkeypatch):
-    """call1DTDSE without snapshots runs the segments of each step side by side (both
-    recurrences truncated at a halo on each side); the persistent kernel (forward recurrence
-    exact across segments) gives the same series to rounding, and both match the reference."""
+    """call1DTDSE runs the segments of each step side by side (both recurrences truncated at a
+    halo on each side); the persistent kernel (forward recurrence exact across segments) gives
+    the same series and wavefunction snapshots to rounding, and both match the reference."""
     g = golden("ref_single.npz")
-    inputs, par = run_case(dll, g)
+    inputs, par = run_case(dll, g, writewft=1, tprint=10.0)
     monkeypatch.setenv("MMA_TDSE_NO_STEP_PARALLEL", "1")
-    inputs2, seq = run_case(dll, g)
+    inputs2, seq = run_case(dll, g, writewft=1, tprint=10.0)
     monkeypatch.delenv("MMA_TDSE_NO_STEP_PARALLEL")
     np.testing.assert_array_equal(par.get_Efield(), seq.get_Efield())
     for k in SERIES:
@@ -92,7 +92,16 @@ def test_single_instance_paths_agree(dll, monkeypatch):
         e = rel_err(a, b)
         print("%-10s parallel vs persistent %.3e" % (k, e), flush=True)
         assert e < 1e-12, k
+    wa = par.get_wavefunction(inputs, grids=False)
+    wb = seq.get_wavefunction(inputs2, grids=False)
+    assert wa.shape == wb.shape and wa.shape[0] == 4416 // 40
+    assert np.abs(wa[-1]).max() > 0  # the last row holds the final state
+    e = max(rel_err(wa[r], wb[r]) for r in range(wa.shape[0]))
+    print("snapshots parallel vs persistent %.3e" % e, flush=True)
+    assert e < 1e-12
     check_series(par, g)
+    dll.free_mtrx(par.psi, len(wa))
+    dll.free_mtrx(seq.psi, len(wb))
     for o in (par, seq):
         o.delete(dll)
     for i in (inputs, inputs2):
