@@ -256,11 +256,11 @@ int hk_gemm(int kind, int m, int n, int k, const float *a, long long lda, const 
     hipStream_t s = (hipStream_t)stream;
     const dim3 grid((unsigned)((n + 127) / 128), (unsigned)((m + 127) / 128));
     if (kind == HK_GEMM_F32)
-        hipLaunchKernelGGL(hk_gemm_kernel<HK_GEMM_F32>, grid, dim3(256), 0, s, m, n, k, a, lda, b, ldb, out, ldo);
+        hipLaunchKernelGGL(hk_gemm_kernel<HK_GEMM_F32>, grid, dim3(512), 0, s, m, n, k, a, lda, b, ldb, out, ldo);
     else if (kind == HK_GEMM_BF16X3)
-        hipLaunchKernelGGL(hk_gemm_kernel<HK_GEMM_BF16X3>, grid, dim3(256), 0, s, m, n, k, a, lda, b, ldb, out, ldo);
+        hipLaunchKernelGGL(hk_gemm_kernel<HK_GEMM_BF16X3>, grid, dim3(512), 0, s, m, n, k, a, lda, b, ldb, out, ldo);
     else if (kind == HK_GEMM_BF16X6)
-        hipLaunchKernelGGL(hk_gemm_kernel<HK_GEMM_BF16X6>, grid, dim3(256), 0, s, m, n, k, a, lda, b, ldb, out, ldo);
+        hipLaunchKernelGGL(hk_gemm_kernel<HK_GEMM_BF16X6>, grid, dim3(512), 0, s, m, n, k, a, lda, b, ldb, out, ldo);
     else
         return fail(HK_ERR_ARGUMENT, "unknown GEMM kind");
     HK_HIP(hipGetLastError());

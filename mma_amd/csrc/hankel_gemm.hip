@@ -1,8 +1,10 @@
 // Real GEMM out[m][n] = sum_k A[m][k] B[n][k] on the CDNA4 matrix cores for the gridded
 // Hankel contraction (A = J0(rho_m Q_l), B = the spread sources, one row per real/imaginary
-// part of each frequency).  Block tile 128 x 128 with four waves of 64 x 64, each wave a 2 x 2
-// grid of 32 x 32 MFMA tiles; K staged through LDS 32 at a time.  Each stage's MFMA sums
-// (FP32) are added into FP64 accumulators, so FP32 rounding grows with 32 terms, not with K.
+// part of each frequency).  Block tile 128 x 128 with eight waves of 32 x 64 (two waves per
+// SIMD fit the registers), each wave a 1 x 2 row of 32 x 32 MFMA tiles; K staged through LDS
+// 32 at a time, the next stage's global loads in flight during the current stage's MFMAs.
+// Every kFold stages the MFMA sums (FP32) are added into FP64 accumulators, so FP32 rounding
+// grows with 64 terms, not with K.
 //   KIND 1: v_mfma_f32_32x32x2_f32 on the FP32 operands.
 //   KIND 2: each FP32 operand split while staging into bf16 x = p0 + p1 (+ 2^-16 |x|);
 //           products p0 q0 + p0 q1 + p1 q0 on v_mfma_f32_32x32x16_bf16.
@@ -19,6 +21,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 namespace {
 constexpr int BM = 128, BN = 128, BK = 32;
+constexpr int kWaves = 8, kThreads = 64 * kWaves;  // 4 x 2 waves of 32 x 64
+constexpr int kLoads = BM * BK / 4 / kThreads;      // float4 loads per thread per operand and stage
+constexpr int kFold = 2;  // K stages summed in FP32 before the FP64 accumulators
 constexpr int kPadF = 1;  // float row padding: conflict-free column reads
 constexpr int kPadH = 8;  // bf16 row padding: keeps 16-byte alignment of 8-element reads
 
@@ -34,7 +39,7 @@ struct FloatTiles {
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kThreads, 2)
 hk_gemm_kernel(int M, int N, int K, const float *A, long long lda, const float *B, long long ldb, double *out,
                long long ldo)
 {
@@ -42,119 +47,121 @@ hk_gemm_kernel(int M, int N, int K, const float *A, long long lda, const float *
     constexpr bool SPLIT = KIND != HK_GEMM_F32;
     __shared__ typename std::conditional<SPLIT, SplitTiles<NP>, FloatTiles>::type T;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave >> 1, wn = wave & 1;  // rows 32 wm .. +32, columns 64 wn .. +64
     const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
-    double accd[2][2][16];
+    double accd[2][16];
 #pragma unroll
-    for (int ti = 0; ti < 2; ti++)
+    for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-        for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) accd[ti][tj][v] = 0.0;
+        for (int v = 0; v < 16; v++) accd[tj][v] = 0.0;
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     const int r32 = lane & 31, kh = lane >> 5;
 
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        float4 ra[4], rb[4];
+    float4 ra[kLoads], rb[kLoads];
+    auto load_stage = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int lin = i * 256 + tid, row = lin >> 3, c4 = lin & 7;
+        for (int i = 0; i < kLoads; i++) {
+            const int lin = i * kThreads + tid, row = lin >> 3, c4 = lin & 7;
             const long long gm = m0 + row, gn = n0 + row;
             ra[i] = gm < M ? *reinterpret_cast<const float4 *>(A + gm * lda + k0 + 4 * c4) : zero4;
             rb[i] = gn < N ? *reinterpret_cast<const float4 *>(B + gn * ldb + k0 + 4 * c4) : zero4;
         }
+    };
+    f32x16 acc[2];
+    auto zero_acc = [&]() {
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[tj][v] = 0.0f;
+    };
+    zero_acc();
+    load_stage(0);
+    for (int k0 = 0, stage = 1; k0 < K; k0 += BK, stage++) {
         __syncthreads();  // the previous tile has been consumed
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int lin = i * 256 + tid, row = lin >> 3, c = 4 * (lin & 7);
-            const float av[4] = {ra[i].x, ra[i].y, ra[i].z, ra[i].w};
-            const float bv[4] = {rb[i].x, rb[i].y, rb[i].z, rb[i].w};
+        for (int i = 0; i < kLoads; i++) {
+            const int lin = i * kThreads + tid, row = lin >> 3, c = 4 * (lin & 7);
+            float xa[4] = {ra[i].x, ra[i].y, ra[i].z, ra[i].w};
+            float xb[4] = {rb[i].x, rb[i].y, rb[i].z, rb[i].w};
+            if constexpr (SPLIT) {
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                if constexpr (SPLIT) {
-                    float xa = av[e], xb = bv[e];
+                for (int q = 0; q < NP; q++) {
+                    union {
+                        __bf16 h[4];
+                        uint2 u;
+                    } pa, pb;
 #pragma unroll
-                    for (int q = 0; q < NP; q++) {
-                        const __bf16 pa = (__bf16)xa, pb = (__bf16)xb;
-                        T.a[q][row][c + e] = pa;
-                        T.b[q][row][c + e] = pb;
-                        xa -= (float)pa;
-                        xb -= (float)pb;
+                    for (int e = 0; e < 4; e++) {
+                        pa.h[e] = (__bf16)xa[e];
+                        pb.h[e] = (__bf16)xb[e];
+                        xa[e] -= (float)pa.h[e];
+                        xb[e] -= (float)pb.h[e];
                     }
-                } else {
-                    T.a[row][c + e] = av[e];
-                    T.b[row][c + e] = bv[e];
+                    *reinterpret_cast<uint2 *>(&T.a[q][row][c]) = pa.u;
+                    *reinterpret_cast<uint2 *>(&T.b[q][row][c]) = pb.u;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    T.a[row][c + e] = xa[e];
+                    T.b[row][c + e] = xb[e];
                 }
             }
         }
         __syncthreads();
-        f32x16 acc[2][2];
-#pragma unroll
-        for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-            for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-                for (int v = 0; v < 16; v++) acc[ti][tj][v] = 0.0f;
+        if (k0 + BK < K) load_stage(k0 + BK);  // in flight during this stage's MFMAs
         if constexpr (SPLIT) {
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
-                bf16x8 ap[NP][2], bp[NP][2];
+                bf16x8 ap[NP], bp[NP][2];
 #pragma unroll
-                for (int q = 0; q < NP; q++)
+                for (int q = 0; q < NP; q++) {
+                    ap[q] = *reinterpret_cast<const bf16x8 *>(&T.a[q][wm * 32 + r32][kk + 8 * kh]);
 #pragma unroll
-                    for (int t = 0; t < 2; t++) {
-                        ap[q][t] = *reinterpret_cast<const bf16x8 *>(&T.a[q][wm * 64 + t * 32 + r32][kk + 8 * kh]);
+                    for (int t = 0; t < 2; t++)
                         bp[q][t] = *reinterpret_cast<const bf16x8 *>(&T.b[q][wn * 64 + t * 32 + r32][kk + 8 * kh]);
-                    }
+                }
 #pragma unroll
-                for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-                    for (int tj = 0; tj < 2; tj++) {
-                        f32x16 c = acc[ti][tj];
-                        if constexpr (NP == 3) {  // smallest terms first
-                            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[2][ti], bp[0][tj], c, 0, 0, 0);
-                            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1][ti], bp[1][tj], c, 0, 0, 0);
-                            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][ti], bp[2][tj], c, 0, 0, 0);
-                        }
-                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1][ti], bp[0][tj], c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][ti], bp[1][tj], c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][ti], bp[0][tj], c, 0, 0, 0);
-                        acc[ti][tj] = c;
+                for (int tj = 0; tj < 2; tj++) {
+                    f32x16 c = acc[tj];
+                    if constexpr (NP == 3) {  // smallest terms first
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[2], bp[0][tj], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1], bp[1][tj], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[2][tj], c, 0, 0, 0);
                     }
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1], bp[0][tj], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[1][tj], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[0][tj], c, 0, 0, 0);
+                    acc[tj] = c;
+                }
             }
         } else {
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 2) {
-                float a[2], b[2];
+                const float a = T.a[wm * 32 + r32][kk + kh];
+                float b[2];
 #pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    a[t] = T.a[wm * 64 + t * 32 + r32][kk + kh];
-                    b[t] = T.b[wn * 64 + t * 32 + r32][kk + kh];
-                }
+                for (int t = 0; t < 2; t++) b[t] = T.b[wn * 64 + t * 32 + r32][kk + kh];
 #pragma unroll
-                for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-                    for (int tj = 0; tj < 2; tj++)
-                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+                for (int tj = 0; tj < 2; tj++) acc[tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b[tj], acc[tj], 0, 0, 0);
             }
         }
-#pragma unroll
-        for (int ti = 0; ti < 2; ti++)
+        if (stage % kFold == 0 || k0 + BK >= K) {
 #pragma unroll
             for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-                for (int v = 0; v < 16; v++) accd[ti][tj][v] += (double)acc[ti][tj][v];
+                for (int v = 0; v < 16; v++) accd[tj][v] += (double)acc[tj][v];
+            zero_acc();
+        }
     }
 #pragma unroll
-    for (int ti = 0; ti < 2; ti++)
+    for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-        for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const long long row = m0 + wm * 64 + ti * 32 + 8 * (v >> 2) + 4 * kh + (v & 3);
-                const long long col = n0 + wn * 64 + tj * 32 + r32;
-                if (row < M && col < N) out[row * ldo + col] = accd[ti][tj][v];
-            }
+        for (int v = 0; v < 16; v++) {
+            const long long row = m0 + wm * 32 + 8 * (v >> 2) + 4 * kh + (v & 3);
+            const long long col = n0 + wn * 64 + tj * 32 + r32;
+            if (row < M && col < N) out[row * ldo + col] = accd[tj][v];
+        }
 }
 
 template __global__ void hk_gemm_kernel<HK_GEMM_F32>(int, int, int, const float *, long long, const float *, long long,
