@@ -126,18 +126,28 @@ int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const 
         return fail(HK_ERR_ARGUMENT, "missing scattering-factor table");
     hipStream_t s = (hipStream_t)stream;
     int *flag = nullptr;
+    double *exit_abs = nullptr;  // per frequency: the z-tabulated absorption at the exit plane
     HK_HIP(hipMalloc(&flag, sizeof(int)));
+    hipError_t e = hipMalloc(&exit_abs, (size_t)no * sizeof(double));
     int zero = 0, got = 0;
-    hipError_t e = hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, s);
     const int n = m->mode == HK_PRESSURE_ZR ? no * nr : no;
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(hk_prefactor_kernel, dim3((n + 127) / 128), dim3(128), 0, s, make_constants(), *m, nz, zgrid,
-                           no, ogrid, nr, rgrid, reinterpret_cast<double2 *>(pre), renorm, flag);
+        // the scalar branch: one thread per (frequency, plane)
+        const unsigned planes = m->mode == HK_PRESSURE_SCALAR ? (unsigned)std::min(nz, 65535) : 1u;
+        hipLaunchKernelGGL(hk_prefactor_kernel, dim3((n + 127) / 128, planes), dim3(128), 0, s, make_constants(), *m,
+                           nz, zgrid, no, ogrid, nr, rgrid, reinterpret_cast<double2 *>(pre), renorm, exit_abs, flag);
+        if (m->mode == HK_PRESSURE_Z) {
+            const long long total = (long long)nz * no;
+            hipLaunchKernelGGL(hk_prefactor_finish_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nz,
+                               no, ogrid, exit_abs, reinterpret_cast<double2 *>(pre), renorm);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&got, flag, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(flag);
+    if (exit_abs) (void)hipFree(exit_abs);
     if (e != hipSuccess) return fail(HK_ERR_DEVICE, std::string("pre-factor kernel: ") + hipGetErrorString(e));
     if (got == HK_FLAG_RANGE) return fail(HK_ERR_TABLE, "photon energy outside the tabulated scattering factors");
     if (got == HK_FLAG_F1) return fail(HK_ERR_TABLE, "f1-scattering coefficient not listed for this frequency.");

@@ -57,8 +57,12 @@ struct Walk {
 
 } // namespace
 
+// One thread per frequency (per (frequency, radius) for the z-r table); the scalar branch
+// spreads its planes over blockIdx.y, and the z-tabulated branch leaves the exponentials to
+// hk_prefactor_finish_kernel (one thread per element) once the exit-plane value is known.
 __global__ void hk_prefactor_kernel(HkConst K, hk_medium m, int nz, const double *zgrid, int no, const double *ogrid,
-                                    int nr, const double *rgrid, double2 *pre, double *renorm, int *flag)
+                                    int nr, const double *rgrid, double2 *pre, double *renorm, double *exit_abs,
+                                    int *flag)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const bool zr = m.mode == HK_PRESSURE_ZR;
@@ -102,7 +106,7 @@ __global__ void hk_prefactor_kernel(HkConst K, hk_medium m, int nz, const double
         const double afac = m.include_absorption ? (p / c) * beta : 0.0;
         const double absw = w * afac;
         const double zl = zgrid[nz - 1];
-        for (int k = 0; k < nz; k++) {
+        for (int k = blockIdx.y; k < nz; k += gridDim.y) {
             const double re = w * ((zgrid[k] - zl) * afac);
             const double im = w * (zgrid[k] * dfac);
             double s, cs;
@@ -191,15 +195,23 @@ __global__ void hk_prefactor_kernel(HkConst K, hk_medium m, int nz, const double
         }
         last_abs = absf;
     }
-    if (zr) return;
-    for (int k = 0; k < nz; k++) {
-        const long long at = (long long)k * no + o;
-        const double2 v = pre[at];
-        const double pm = renorm[at];
-        double s, cs;
-        sincos(w * v.y, &s, &cs);
-        const double e = exp(w * (v.x - last_abs));
-        pre[at] = make_double2(pm * (e * cs), pm * (e * s));
-        renorm[at] = exp(w * last_abs - w * v.x);  // :226-228
-    }
+    if (!zr) exit_abs[o] = last_abs;
+}
+
+// The z-tabulated branch's exponentials from the stash (absorption, dispersion phase) in pre
+// and the pressure in renorm, renormalised to the exit plane (:215, :226-228).
+__global__ void hk_prefactor_finish_kernel(int nz, int no, const double *ogrid, const double *exit_abs, double2 *pre,
+                                           double *renorm)
+{
+    const long long at = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= (long long)nz * no) return;
+    const int o = (int)(at % no);
+    const double w = ogrid[o], last_abs = exit_abs[o];
+    const double2 v = pre[at];
+    const double pm = renorm[at];
+    double s, cs;
+    sincos(w * v.y, &s, &cs);
+    const double e = exp(w * (v.x - last_abs));
+    pre[at] = make_double2(pm * (e * cs), pm * (e * s));
+    renorm[at] = exp(w * last_abs - w * v.x);
 }
