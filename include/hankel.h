@@ -97,10 +97,11 @@ int hk_grid_size(double q_max, double rho_max, double tol, int *nq, double *h);
 
 enum hk_operand { HK_F64 = 0, HK_F32 = 1 };
 
-/* C operand, "B^T" layout: row n = 2 o (real parts) / 2 o + 1 (imaginary parts), nq
- * columns, row stride ld >= nq, starting at row `row0`; type HK_F64 (double) or HK_F32
- * (float).  weights: nz plane weights (zero weights skip the plane). */
-int hk_grid_spread(const hk_planes *p, const double *weights, int nq, double h, int type, void *out,
+/* C operand, "B^T" layout: for output q < n_out, row row0 + 2 (no q + o) holds the real
+ * parts and the next row the imaginary parts of frequency o; nq columns, row stride ld >= nq;
+ * type HK_F64 (double) or HK_F32 (float).  weights: n_out x nz plane weights (zero weights
+ * skip the plane).  All outputs are spread in one launch. */
+int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq, double h, int type, void *out,
                    long long ld, long long row0, void *stream);
 
 /* A operand: J0(rho_m |Q_l|), nrho rows x nq (row stride ld), HK_F64 or HK_F32. */

@@ -205,12 +205,13 @@ int hk_grid_size(double q_max, double rho_max, double tol, int *nq, double *h)
     return HK_OK;
 }
 
-int hk_grid_spread(const hk_planes *p, const double *weights, int nq, double h, int type, void *out, long long ld,
-                   long long row0, void *stream)
+int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq, double h, int type, void *out,
+                   long long ld, long long row0, void *stream)
 {
     clear_status();
     if (check_planes(p) != HK_OK) return g_status;
-    if (!weights || !out || nq < HK_GRID_P + 2 || nq > kMaxBins || ld < nq || row0 < 0 || !(h > 0))
+    if (n_out < 1 || n_out > 65535 || !weights || !out || nq < HK_GRID_P + 2 || nq > kMaxBins || ld < nq ||
+        row0 < 0 || !(h > 0))
         return fail(HK_ERR_ARGUMENT, "bad spread arguments");
     if (p->pre_mode == 2) return fail(HK_ERR_UNSUPPORTED, "the gridded contraction needs an r-independent pre-factor");
     HostGrids g;
@@ -223,13 +224,13 @@ int hk_grid_spread(const hk_planes *p, const double *weights, int nq, double h, 
     if (type == HK_F64) {
         HK_HIP(hipFuncSetAttribute((const void *)hk_spread_kernel<HK_F64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
-        hipLaunchKernelGGL(hk_spread_kernel<HK_F64>, dim3(p->no), dim3(1024), lds, s, *p, weights, nq, h, kInvC, out,
-                           ld, row0);
+        hipLaunchKernelGGL(hk_spread_kernel<HK_F64>, dim3(p->no, n_out), dim3(1024), lds, s, *p, weights, nq, h, kInvC,
+                           out, ld, row0);
     } else if (type == HK_F32) {
         HK_HIP(hipFuncSetAttribute((const void *)hk_spread_kernel<HK_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
-        hipLaunchKernelGGL(hk_spread_kernel<HK_F32>, dim3(p->no), dim3(1024), lds, s, *p, weights, nq, h, kInvC, out,
-                           ld, row0);
+        hipLaunchKernelGGL(hk_spread_kernel<HK_F32>, dim3(p->no, n_out), dim3(1024), lds, s, *p, weights, nq, h, kInvC,
+                           out, ld, row0);
     } else {
         return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");
     }

@@ -173,8 +173,9 @@ __device__ inline void store_operand(void *out, long long at, double v)
 
 } // namespace
 
-// one workgroup per frequency; the nq bins of C(., omega) accumulate in LDS (FP64 atomics,
-// real and imaginary parts in separate arrays: fewer bank conflicts than interleaved pairs).
+// one workgroup per (frequency, output); the nq bins of C(., omega) accumulate in LDS (FP64
+// atomics, real and imaginary parts in separate arrays: fewer bank conflicts than interleaved
+// pairs).
 // The (plane, radius) pairs of the planes with non-zero weight are dealt to the 1024 threads
 // in one flat range, advanced with 32-bit counters, so every wave has work whatever nr is.
 template <int TYPE>
@@ -186,6 +187,8 @@ hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double in
     __shared__ int k_range[2];
     double *acc_re = bins, *acc_im = bins + nq;
     const int o = blockIdx.x;
+    weights += (long long)blockIdx.y * p.nz;  // output blockIdx.y
+    row0 += 2LL * p.no * blockIdx.y;
     for (int l = threadIdx.x; l < 2 * nq; l += blockDim.x) bins[l] = 0.0;
     if (threadIdx.x == 0) {
         k_range[0] = p.nz;

@@ -68,8 +68,8 @@ def _L():
                                    c_void_p, c_void_p]
         L.hk_direct.argtypes = [POINTER(hk_planes), c_int, c_void_p, c_int, c_void_p, c_void_p]
         L.hk_grid_size.argtypes = [c_double, c_double, c_double, POINTER(c_int), POINTER(c_double)]
-        L.hk_grid_spread.argtypes = [POINTER(hk_planes), c_void_p, c_int, c_double, c_int, c_void_p, c_longlong,
-                                     c_longlong, c_void_p]
+        L.hk_grid_spread.argtypes = [POINTER(hk_planes), c_int, c_void_p, c_int, c_double, c_int, c_void_p,
+                                     c_longlong, c_longlong, c_void_p]
         L.hk_grid_bessel.argtypes = [c_int, c_void_p, c_int, c_double, c_int, c_void_p, c_longlong, c_void_p]
         L.hk_gemm.argtypes = [c_int, c_int, c_int, c_int, c_void_p, c_longlong, c_void_p, c_longlong, c_void_p,
                               c_longlong, c_void_p]
@@ -336,9 +336,8 @@ def gridded(src, zgrid, ogrid, rgrid, rho, distance, weights, near_field=True, p
     otype = HK_F64 if kind == "fp64-grid" else HK_F32
     dt = torch.float64 if otype == HK_F64 else torch.float32
     Bm = torch.empty((n_out * 2 * p.no, nq), dtype=dt, device="cuda")
-    for q in range(n_out):
-        wq = _dev(W[q])
-        _check(L.hk_grid_spread(byref(p), _ptr(wq), nq, h, otype, _ptr(Bm), nq, q * 2 * p.no, _stream()))
+    wq = _dev(np.ascontiguousarray(W))
+    _check(L.hk_grid_spread(byref(p), n_out, _ptr(wq), nq, h, otype, _ptr(Bm), nq, 0, _stream()))
     rh = _dev(rho)
     Am = torch.empty((p.nrho, nq), dtype=dt, device="cuda")
     _check(L.hk_grid_bessel(p.nrho, _ptr(rh), nq, h, otype, _ptr(Am), nq, _stream()))
