@@ -9,13 +9,17 @@ propagation of the whole batch (one persistent kernel launch).
 value = total grid-point-steps per second (sum over instances of Nx * steps / wall time),
 inputs resident in HBM.  The JSON line also carries the HBM roofline of the propagation
 kernel (32 B of psi traffic per grid-point-step, HIP events on the launch stream), the
-per-instance rate (one instance alone), the gradient workload's adjoint/forward time ratio
-(1024 instances, Nx = 8001, dt = 0.1, 40000 field samples) and a CPU baseline: the
-reference's own CTDSE sources compiled in oracle/_ref (else the FP64 oracle) on a bounded
-sample of whole members, one per host thread.
+per-instance rate (one instance alone, as call1DTDSE runs it), the 8192-member uneven batch,
+the one-launch-per-step form of the headline batch, the gradient workload's adjoint/forward
+time ratio (1024 instances, Nx = 8001, dt = 0.1, 40000 field samples), the far-field
+contraction (256 planes x 2048 frequencies x 512 radii -> 1024 screen radii) and a CPU
+baseline: the reference's own CTDSE sources compiled in oracle/_ref (else the FP64 oracle)
+on a bounded sample of whole members, one per host thread.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-single]
-                       [--no-adjoint] [--only-adjoint] [--copy-probe MIB]
+                       [--no-adjoint] [--no-uneven] [--no-stepwise] [--no-hankel]
+                       [--only-single | --only-uneven | --only-adjoint | --only-hankel]
+                       [--launch-config W,G,S] [--stream-probe GIB] [--copy-probe MIB]
 """
 import argparse
 import json
