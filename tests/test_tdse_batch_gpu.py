@@ -107,19 +107,27 @@ def test_full_gaussian_beam_batch_invariants():
 def test_propagate_one_matches_batch_rows():
     """ctdse_propagate_one (segments of a step side by side, a halo on both sides of each)
     against the batch rows of the same instances, on grids of one segment up to many and
-    with steps fewer than the field holds: equal to rounding; the Efield row bitwise."""
+    with steps fewer than the field holds (down to one step: entry 0 alone, whose initial
+    values are exact): equal to rounding; the Efield row bitwise."""
     rng = np.random.Generator(np.random.PCG64(11))
     for num_r, dx in ((300, 0.2), (2000, 0.1), (16000, 0.1), (9001, 0.05)):
         geo = tdse.Geometry(num_r, dx, 0.25, W.GRID["a"], W.GRID["e_guess"], 1e-25, 2.0)
         tt = np.arange(700) * 0.25
         f = W.sin2_pulse(tt, 0.08, W.omega_au(800e-9), 3, phase=rng.uniform(0, 3))
         dev = torch.tensor(tdse.pad_fields(f), device="cuda")
-        for n_steps in (dev.shape[1] - 1, 401):
+        for n_steps in (dev.shape[1] - 1, 401, 3, 2, 1):
             ref = tdse.propagate_batch(geo, dev, n_steps=[n_steps])[0]
             got = tdse.propagate_one(geo, dev[0].contiguous(), n_steps)
             torch.cuda.synchronize()
             got, ref = got.cpu().numpy(), ref.cpu().numpy()
             np.testing.assert_array_equal(got[4], ref[4])
+            if n_steps <= 3:
+                # the pulse has barely started: the state is still near the ground state, whose
+                # dipole and <x> vanish to rounding, so the rows are compared on the scale of
+                # PopTot (1); entry 0 holds the exact initial values in both
+                np.testing.assert_array_equal(got[:, 0], ref[:, 0])
+                np.testing.assert_allclose(got[:4], ref[:4], rtol=0, atol=1e-12)
+                continue
             for r, k in enumerate(ROWS[:4]):
                 e = rel_err(got[r], ref[r])
                 print(num_r, n_steps, k, "%.2e" % e, flush=True)
