@@ -468,7 +468,26 @@ int pick_waves(int *min_out, int max_halo_lanes)
 // chunks) and resident workgroups per CU (the persistent grid is CUs * per_cu).
 struct LaunchShape {
     int nw = 4, per_cu = 3, steps = 1;
+    bool stash = false;  // two-step passes keep 1/beta in LDS between their stages
 };
+
+// Dynamic LDS of tdse_propagate_pair_stash_kernel: two segments' 1/beta.
+size_t stash_bytes(int nw) { return 2 * 64 * (size_t)nw * TDSE_CHUNK * sizeof(double2); }
+
+// Whether per_cu workgroups of the stash kernel (static LDS + stash) fit one CU's LDS, so
+// that the shape keeps its occupancy (MMA_TDSE_STASH=0 turns the stash off).
+bool stash_fits(const LaunchShape &sh)
+{
+    if (const char *env = std::getenv("MMA_TDSE_STASH"))
+        if (std::atoi(env) == 0) return false;
+    hipFuncAttributes attr;
+    int dev = 0, lds = 0;
+    if (hipFuncGetAttributes(&attr, (const void *)tdse_propagate_pair_stash_kernel) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+        return false;
+    return (size_t)sh.per_cu * (attr.sharedSizeBytes + stash_bytes(sh.nw)) <= (size_t)lds;
+}
 
 int shape_for(const std::vector<const ctdse_geometry *> &geoms, const ctdse_launch_config *cfg, LaunchShape *out)
 {
@@ -492,6 +511,10 @@ int shape_for(const std::vector<const ctdse_geometry *> &geoms, const ctdse_laun
         sh.per_cu = cfg->groups_per_cu;
         sh.steps = cfg->steps_per_pass;
     }
+    sh.stash = sh.steps == 2 && stash_fits(sh);
+    if (sh.stash)  // outside any stream capture: plans and immediate runs call this first
+        HIP_CHECK(hipFuncSetAttribute((const void *)tdse_propagate_pair_stash_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)stash_bytes(sh.nw)));
     *out = sh;
     return CTDSE_OK;
 }
@@ -600,9 +623,12 @@ int grid_slots(int n, const LaunchShape &sh, int *slots)
 int enqueue_plan(const ctdse_batch_plan &P, double2 *work, hipStream_t stream)
 {
     hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, P.d_counter.as<int>(), 1, 0);
-    hipLaunchKernelGGL(P.shape.steps == 2 ? tdse_propagate_pair_kernel : tdse_propagate_kernel, dim3(P.slots),
-                       dim3(64 * P.shape.nw), 0, stream, P.d_geom.as<TdseGeom>(), P.d_inst.as<TdseInst>(),
-                       P.d_order.as<int>(), P.n, P.d_counter.as<int>(), work, P.stride);
+    const bool stash = P.shape.steps == 2 && P.shape.stash;
+    hipLaunchKernelGGL(stash ? tdse_propagate_pair_stash_kernel
+                             : (P.shape.steps == 2 ? tdse_propagate_pair_kernel : tdse_propagate_kernel),
+                       dim3(P.slots), dim3(64 * P.shape.nw), stash ? stash_bytes(P.shape.nw) : 0, stream,
+                       P.d_geom.as<TdseGeom>(), P.d_inst.as<TdseInst>(), P.d_order.as<int>(), P.n,
+                       P.d_counter.as<int>(), work, P.stride);
     HIP_CHECK(hipGetLastError());
     return CTDSE_OK;
 }

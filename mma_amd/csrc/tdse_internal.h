@@ -142,6 +142,8 @@ __global__ void tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *ins
                                       int *counter, double2 *work, long long work_stride);
 __global__ void tdse_propagate_pair_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
                                            int *counter, double2 *work, long long work_stride);
+__global__ void tdse_propagate_pair_stash_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order,
+                                                 int n_inst, int *counter, double2 *work, long long work_stride);
 __global__ void tdse_untile_kernel(const double2 *psi, int nx, double *row);
 __global__ void tdse_parallel_step_kernel(const TdseGeom *geom, const TdseInst *insts, int k, const double2 *psi_in,
                                           double2 *psi_out, long long state_stride, double *partial, int nseg);

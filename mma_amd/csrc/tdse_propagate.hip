@@ -73,7 +73,10 @@ struct SegTables {
     double gL, gR;   // g left of the chunk (lane 0) and right of it (lane 63)
 };
 
-__device__ inline void load_tables(const TdseGeom &G, int cs, int wave, int lane, int nw, SegTables &T)
+// IB_LDS: 1/beta from `stash` (point i of lane gl at stash[i * 64 nw + gl]) instead of HBM.
+template <bool IB_LDS = false>
+__device__ inline void load_tables(const TdseGeom &G, int cs, int wave, int lane, int nw, SegTables &T,
+                                   const double2 *stash = nullptr)
 {
     const int nwl = min(nw, (G.nchunks - cs + 63) >> 6);
     const int gl = wave * 64 + lane;
@@ -83,7 +86,10 @@ __device__ inline void load_tables(const TdseGeom &G, int cs, int wave, int lane
 #pragma unroll
     for (int i = 0; i < C; i++) {
         T.Gv[i] = ld1(G.g, base + 64 * i);
-        T.IB[i] = ld2(G.ib, base + 64 * i);
+        if constexpr (IB_LDS)
+            T.IB[i] = stash[i * 64 * nw + gl];
+        else
+            T.IB[i] = ld2(G.ib, base + 64 * i);
         T.dp[i] = ld1(G.dpot, base + 64 * i);
     }
     T.Qc = ld2(G.chq, cl + 64);
@@ -119,11 +125,14 @@ enum SegMode { kSingle = 0, kKeep = 1, kFromRegs = 2, kParallel = 3 };
 // edge_in / edge_out: {old psi, forward z} at the last own point of the previous / this
 // segment (LDS, alternating between segments).  par: parity of the segment (PairLds
 // buffers).
-template <int MODE>
+// STASH (pair passes): the first stage (kKeep) leaves the segment's 1/beta in `stash` (LDS),
+// where the second stage (kFromRegs) of the same segment reads it instead of fetching it again.
+template <int MODE, bool STASH = false>
 __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wave, int lane, int nw,
                               double2 *__restrict__ psi, double2 e0, double2 rot, bool observe, Acc &acc,
                               double2 (*slot_f)[2], double2 (*slot_b)[2], const double2 *edge_in, double2 *edge_out,
-                              double2 (&R)[C], PairLds *pl, double2 *__restrict__ psi_out = nullptr)
+                              double2 (&R)[C], PairLds *pl, double2 *__restrict__ psi_out = nullptr,
+                              double2 *stash = nullptr)
 {
     const int gl = wave * 64 + lane;                 // lane index in the workgroup
     // first chunk of the window: the first own chunk, or H chunks before it (kParallel)
@@ -145,7 +154,15 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     const int par = seg & 1;
 
     SegTables T;
-    load_tables(G, cs, wave, lane, nw, T);
+    if constexpr (STASH && MODE == kFromRegs) {
+        load_tables<true>(G, cs, wave, lane, nw, T, stash);
+    } else {
+        load_tables(G, cs, wave, lane, nw, T);
+        if constexpr (STASH && MODE == kKeep) {
+#pragma unroll
+            for (int i = 0; i < C; i++) stash[i * 64 * nw + gl] = T.IB[i];
+        }
+    }
     double2 P[C];
     const double2 zero2 = make_double2(0.0, 0.0);
     double2 nb = zero2;  // lane 0: the point left of the wave's first chunk; lane 63: right of its last
@@ -378,10 +395,10 @@ __device__ inline StepPhase step_phase(const TdseGeom &G, double F, int wave, in
 // LDS) and stored.  psi then streams through HBM once per two steps.
 // States saved for partial runs and snapshots are copied from the finished state between
 // steps (they are rare), so the segment solve carries no extra stores.
-template <bool PAIR>
+template <bool PAIR, bool STASH = false>
 __device__ void propagate_loop(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
                                const int *__restrict__ order, int n_inst, int *counter, double2 *work,
-                               long long work_stride, PairLds *pl)
+                               long long work_stride, PairLds *pl, double2 *stash = nullptr)
 {
     __shared__ double part[2][TDSE_MAX_WAVES][5];
     __shared__ double2 slot_f[TDSE_MAX_WAVES][2], slot_b[TDSE_MAX_WAVES][2];
@@ -440,25 +457,33 @@ __device__ void propagate_loop(const TdseGeom *__restrict__ geoms, const TdseIns
                 Acc acc0 = {0.0, 0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0, 0.0};
                 double2 e0 = ph0.e_lane, e1 = ph1.e_lane;
                 // segment s of step k into R(s&1), then segment s-1 of step k+1 from R((s-1)&1)
+                // (STASH: segment s's 1/beta in stash buffer s & 1 from the first stage to the second)
+                const int stash_len = 64 * nw * C;
                 for (int s = 0; s <= nseg; s++) {
                     if (s < nseg) {
+                        double2 *st = STASH ? stash + (s & 1) * stash_len : nullptr;
                         if (s & 1)
-                            solve_segment<kKeep>(G, s, own_chunks, wave, lane, nw, buf, e0, ph0.rot, observe0, acc0,
-                                                 slot_f, slot_b, edge[s & 1], edge[(s + 1) & 1], R1, pl);
+                            solve_segment<kKeep, STASH>(G, s, own_chunks, wave, lane, nw, buf, e0, ph0.rot, observe0,
+                                                        acc0, slot_f, slot_b, edge[s & 1], edge[(s + 1) & 1], R1, pl,
+                                                        nullptr, st);
                         else
-                            solve_segment<kKeep>(G, s, own_chunks, wave, lane, nw, buf, e0, ph0.rot, observe0, acc0,
-                                                 slot_f, slot_b, edge[s & 1], edge[(s + 1) & 1], R0, pl);
+                            solve_segment<kKeep, STASH>(G, s, own_chunks, wave, lane, nw, buf, e0, ph0.rot, observe0,
+                                                        acc0, slot_f, slot_b, edge[s & 1], edge[(s + 1) & 1], R0, pl,
+                                                        nullptr, st);
                         e0 = cmul(e0, ph0.r_seg);
                     }
                     lds_barrier();  // the halo and wave-edge points of segment s are in LDS
                     if (s > 0) {
                         const int sb = s - 1;
+                        double2 *st = STASH ? stash + (sb & 1) * stash_len : nullptr;
                         if (sb & 1)
-                            solve_segment<kFromRegs>(G, sb, own_chunks, wave, lane, nw, buf, e1, ph1.rot, true, acc1,
-                                                     slot_f, slot_b, edge_b[sb & 1], edge_b[(sb + 1) & 1], R1, pl);
+                            solve_segment<kFromRegs, STASH>(G, sb, own_chunks, wave, lane, nw, buf, e1, ph1.rot, true,
+                                                            acc1, slot_f, slot_b, edge_b[sb & 1],
+                                                            edge_b[(sb + 1) & 1], R1, pl, nullptr, st);
                         else
-                            solve_segment<kFromRegs>(G, sb, own_chunks, wave, lane, nw, buf, e1, ph1.rot, true, acc1,
-                                                     slot_f, slot_b, edge_b[sb & 1], edge_b[(sb + 1) & 1], R0, pl);
+                            solve_segment<kFromRegs, STASH>(G, sb, own_chunks, wave, lane, nw, buf, e1, ph1.rot, true,
+                                                            acc1, slot_f, slot_b, edge_b[sb & 1],
+                                                            edge_b[(sb + 1) & 1], R0, pl, nullptr, st);
                         e1 = cmul(e1, ph1.r_seg);
                     }
                 }
@@ -577,6 +602,18 @@ tdse_propagate_pair_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *_
 {
     __shared__ PairLds pl;
     propagate_loop<true>(geoms, insts, order, n_inst, counter, work, work_stride, &pl);
+}
+
+// The same, with each segment's 1/beta kept in LDS between the two stages of a pass (dynamic
+// LDS of 2 * 64 * waves * 8 complex doubles): the second stage does not fetch it again.
+__global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_PAIR_WAVES_PER_SIMD)
+tdse_propagate_pair_stash_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
+                                 const int *__restrict__ order, int n_inst, int *counter, double2 *work,
+                                 long long work_stride)
+{
+    __shared__ PairLds pl;
+    extern __shared__ double2 stash[];
+    propagate_loop<true, true>(geoms, insts, order, n_inst, counter, work, work_stride, &pl, stash);
 }
 
 // One time step k of instance blockIdx.y, split over independent segments (blockIdx.x) whose
