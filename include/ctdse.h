@@ -225,7 +225,8 @@ int ctdse_geometry_create_batch(int n, const int *num_r, const double *dx, const
  * of the parallel tridiagonal solve covers 64 * waves_per_group chunks of 8 points),
  * resident workgroups per compute unit, and time steps per pass over the state (1, or 2:
  * the intermediate state stays on chip wherever no save or snapshot falls between the
- * two steps). */
+ * two steps; where groups_per_cu workgroups still fit a CU's LDS with it, each segment's
+ * 1/beta also stays in LDS from the first step to the second). */
 typedef struct ctdse_launch_config {
     int waves_per_group;
     int groups_per_cu;
