@@ -220,20 +220,23 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
     if (q_max / h > (double)(nq - HK_GRID_P - 1) * (1.0 + 1e-9))
         return fail(HK_ERR_ARGUMENT, "q grid too short for these sources (use hk_grid_size)");
     hipStream_t s = (hipStream_t)stream;
-    const size_t lds = (size_t)nq * sizeof(double2);
-    if (type == HK_F64) {
-        HK_HIP(hipFuncSetAttribute((const void *)hk_spread_kernel<HK_F64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-        hipLaunchKernelGGL(hk_spread_kernel<HK_F64>, dim3(p->no, n_out), dim3(1024), lds, s, *p, weights, nq, h, kInvC,
-                           out, ld, row0);
-    } else if (type == HK_F32) {
-        HK_HIP(hipFuncSetAttribute((const void *)hk_spread_kernel<HK_F32>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-        hipLaunchKernelGGL(hk_spread_kernel<HK_F32>, dim3(p->no, n_out), dim3(1024), lds, s, *p, weights, nq, h, kInvC,
-                           out, ld, row0);
-    } else {
+    // bins, then (when they fit the LDS too) 5 doubles of per-plane factors
+    const size_t lds_bins = (size_t)nq * sizeof(double2), lds_tab = lds_bins + (size_t)p->nz * 5 * sizeof(double);
+    const bool tab = lds_tab <= (size_t)kMaxBins * sizeof(double2);
+    const size_t lds = tab ? lds_tab : lds_bins;
+    auto launch = [&](auto kernel) -> int {
+        HK_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kernel, dim3(p->no, n_out), dim3(1024), lds, s, *p, weights, nq, h, kInvC, out, ld, row0);
+        return HK_OK;
+    };
+    int st;
+    if (type == HK_F64)
+        st = tab ? launch(hk_spread_kernel<HK_F64, true>) : launch(hk_spread_kernel<HK_F64, false>);
+    else if (type == HK_F32)
+        st = tab ? launch(hk_spread_kernel<HK_F32, true>) : launch(hk_spread_kernel<HK_F32, false>);
+    else
         return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");
-    }
+    if (st != HK_OK) return st;
     HK_HIP(hipGetLastError());
     HK_HIP(hipStreamSynchronize(s));
     return HK_OK;

@@ -178,32 +178,47 @@ __device__ inline void store_operand(void *out, long long at, double v)
 // pairs).
 // The (plane, radius) pairs of the planes with non-zero weight are dealt to the 1024 threads
 // in one flat range, advanced with 32-bit counters, so every wave has work whatever nr is.
-template <int TYPE>
+// TAB: the per-plane factors (K / D_k / h, -K / (2 D_k), w_k, pre_k) sit in LDS after the
+// bins (5 doubles per plane), so an item does no division and no pre-factor load.
+template <int TYPE, bool TAB>
 __global__ void __launch_bounds__(kSpreadThreads)
 hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double inv_c, void *out, long long ld,
                  long long row0)
 {
     extern __shared__ double bins[];
     __shared__ int k_range[2];
-    double *acc_re = bins, *acc_im = bins + nq;
+    double *acc_re = bins, *acc_im = bins + nq, *tab = bins + 2 * nq;
     const int o = blockIdx.x;
     weights += (long long)blockIdx.y * p.nz;  // output blockIdx.y
     row0 += 2LL * p.no * blockIdx.y;
+    const double K = p.ogrid[o] * inv_c;
+    const double inv_h = 1.0 / h;
     for (int l = threadIdx.x; l < 2 * nq; l += blockDim.x) bins[l] = 0.0;
     if (threadIdx.x == 0) {
         k_range[0] = p.nz;
         k_range[1] = -1;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < p.nz; k += blockDim.x)
-        if (weights[k] != 0.0) {
+    for (int k = threadIdx.x; k < p.nz; k += blockDim.x) {
+        const double wk = weights[k];
+        if (wk != 0.0) {
             atomicMin(&k_range[0], k);
             atomicMax(&k_range[1], k);
         }
+        if constexpr (TAB) {
+            const double D = p.distance - p.zgrid[k];
+            double2 pk = make_double2(1.0, 0.0);
+            if (p.pre_mode == 1) pk = reinterpret_cast<const double2 *>(p.pre)[(long long)k * p.no + o];
+            double *t = tab + 5 * k;
+            t[0] = (K / D) * inv_h;
+            t[1] = -K / (2.0 * D);
+            t[2] = wk;
+            t[3] = pk.x;
+            t[4] = pk.y;
+        }
+    }
     __syncthreads();
     const int k_end = k_range[1] + 1;
-    const double K = p.ogrid[o] * inv_c;
-    const double inv_h = 1.0 / h;
     const bool c64 = p.src_c64 != 0;
     const int step_k = blockDim.x / p.nr, step_j = blockDim.x - step_k * p.nr;
     int k = k_range[0] + threadIdx.x / p.nr, j = threadIdx.x % p.nr;
@@ -212,19 +227,41 @@ hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double in
             j -= p.nr;
             if (++k >= k_end) break;
         }
-        const double wk = weights[k];
-        if (wk == 0.0) continue;
-        const double D = p.distance - p.zgrid[k];
-        const double s = K / D;
-        double2 pk = make_double2(1.0, 0.0);
-        if (p.pre_mode == 1) pk = reinterpret_cast<const double2 *>(p.pre)[(long long)k * p.no + o];
-        double2 v = c64 ? source_term(p, reinterpret_cast<const float2 *>(p.src) + (long long)k * p.plane_stride, k,
-                                      o, j, K, D, pk)
-                        : source_term(p, reinterpret_cast<const double2 *>(p.src) + (long long)k * p.plane_stride, k,
-                                      o, j, K, D, pk);
-        v.x *= wk;
-        v.y *= wk;
-        const double t = s * p.rgrid[j] * inv_h;
+        double2 v;
+        double t;
+        if constexpr (TAB) {
+            const double *tk = tab + 5 * k;
+            const double wk = tk[2];
+            if (wk == 0.0) continue;
+            const double r = p.rgrid[j];
+            const long long at = (long long)k * p.plane_stride + (long long)o * p.nr + j;
+            const double2 f = c64 ? load_src(reinterpret_cast<const float2 *>(p.src), at)
+                                  : load_src(reinterpret_cast<const double2 *>(p.src), at);
+            v = cmul(make_double2(tk[3], tk[4]), f);
+            if (p.near_field) {
+                double sn, cs;
+                sincos(tk[1] * (r * r), &sn, &cs);
+                v = cmul(make_double2(cs, sn), v);
+            }
+            const double wr = (p.tw[j] * r) * wk;
+            v.x *= wr;
+            v.y *= wr;
+            t = tk[0] * r;
+        } else {
+            const double wk = weights[k];
+            if (wk == 0.0) continue;
+            const double D = p.distance - p.zgrid[k];
+            const double s = K / D;
+            double2 pk = make_double2(1.0, 0.0);
+            if (p.pre_mode == 1) pk = reinterpret_cast<const double2 *>(p.pre)[(long long)k * p.no + o];
+            v = c64 ? source_term(p, reinterpret_cast<const float2 *>(p.src) + (long long)k * p.plane_stride, k, o, j,
+                                  K, D, pk)
+                    : source_term(p, reinterpret_cast<const double2 *>(p.src) + (long long)k * p.plane_stride, k, o,
+                                  j, K, D, pk);
+            v.x *= wk;
+            v.y *= wk;
+            t = s * p.rgrid[j] * inv_h;
+        }
         const double b = floor(t);
         const Lagrange lw(t - b);
         const int l0 = (int)b + 1;  // bin of node offset -(P/2 - 1): Q_l = (l - P/2) h
@@ -243,10 +280,14 @@ hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double in
     }
 }
 
-template __global__ void hk_spread_kernel<HK_F64>(hk_planes, const double *, int, double, double, void *, long long,
-                                                  long long);
-template __global__ void hk_spread_kernel<HK_F32>(hk_planes, const double *, int, double, double, void *, long long,
-                                                  long long);
+template __global__ void hk_spread_kernel<HK_F64, false>(hk_planes, const double *, int, double, double, void *,
+                                                         long long, long long);
+template __global__ void hk_spread_kernel<HK_F32, false>(hk_planes, const double *, int, double, double, void *,
+                                                         long long, long long);
+template __global__ void hk_spread_kernel<HK_F64, true>(hk_planes, const double *, int, double, double, void *,
+                                                        long long, long long);
+template __global__ void hk_spread_kernel<HK_F32, true>(hk_planes, const double *, int, double, double, void *,
+                                                        long long, long long);
 
 // J0(rho_m |Q_l|), Q_l = (l - P/2) h
 template <int TYPE>

@@ -27,7 +27,7 @@ template <typename S>
 __global__ void hk_direct_kernel(hk_planes p, int n_out, const double *weights, int per_plane, double2 *out,
                                  double inv_c, int uniform_r, double r0, double dr);
 
-template <int TYPE>
+template <int TYPE, bool TAB>
 __global__ void hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double inv_c, void *out,
                                  long long ld, long long row0);
 
