@@ -60,10 +60,24 @@ struct Walk {
 // One thread per frequency (per (frequency, radius) for the z-r table); the scalar branch
 // spreads its planes over blockIdx.y, and the z-tabulated branch leaves the exponentials to
 // hk_prefactor_finish_kernel (one thread per element) once the exit-plane value is known.
+// staged (z-tabulated branch): the plane grid and the pressure table are first copied to
+// dynamic LDS (nz + 2 np doubles), so the serial walk of each thread reads no global memory.
 __global__ void hk_prefactor_kernel(HkConst K, hk_medium m, int nz, const double *zgrid, int no, const double *ogrid,
                                     int nr, const double *rgrid, double2 *pre, double *renorm, double *exit_abs,
-                                    int *flag)
+                                    int *flag, int staged)
 {
+    extern __shared__ double stage[];
+    if (staged) {
+        for (int i = threadIdx.x; i < nz; i += blockDim.x) stage[i] = zgrid[i];
+        for (int i = threadIdx.x; i < m.np_z; i += blockDim.x) {
+            stage[nz + i] = m.p_z[i];
+            stage[nz + m.np_z + i] = m.p_value[i];
+        }
+        __syncthreads();
+        zgrid = stage;
+        m.p_z = stage + nz;
+        m.p_value = stage + nz + m.np_z;
+    }
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const bool zr = m.mode == HK_PRESSURE_ZR;
     if (tid >= (zr ? no * nr : no)) return;

@@ -187,6 +187,17 @@ class FSources_provider:
         return torch.stack([cast(torch.as_tensor(np.ascontiguousarray(p), device="cuda")) for p in self.Fsource_plane])
 
 
+_DEV_TABLES = {}
+
+
+def _dev_table(t, key):
+    """Device copy of one XUV table column, uploaded once per process and device."""
+    dev = torch.cuda.current_device()
+    if (dev, key) not in _DEV_TABLES:
+        _DEV_TABLES[(dev, key)] = _dev(t[key])
+    return _DEV_TABLES[(dev, key)]
+
+
 class _PreFactor:
     """The tables behind get_propagation_pre_factor_function: pre (device, complex
     (Nz, No) or (Nz, No, Nr)), renorm (device (Nz, No) or None)."""
@@ -232,7 +243,7 @@ class _PreFactor:
             t = _xuv_table(None)
             if gas + "/" + ek not in t:
                 raise KeyError(gas)
-            e, f = _dev(t[gas + "/" + ek]), _dev(t[gas + "/" + fk])
+            e, f = _dev_table(t, gas + "/" + ek), _dev_table(t, gas + "/" + fk)
             keep += [e, f]
             setattr(med, "n_f" + ns, len(e))
             setattr(med, "e_f" + ns, _ptr(e))
