@@ -233,7 +233,7 @@ def test_gpu_gradient_workload_finite_differences_64_directions():
     dt = 0.1), one seeded random unit direction each (zero up to the gating point): central
     differences (h = 1e-6) of the harmonic yield over harmonics 21-31 against the adjoint
     directional derivative.  The error is measured against |grad| (|direction| = 1), so a
-    direction nearly orthogonal to the gradient is not judged by its own tiny derivative.
+    direction nearly orthogonal to the gradient is not measured against its own tiny derivative.
     Members 0 and 20 are also compared with the oracle gradient over the full length."""
     from mma_amd import adjoint as AD, tdse, workloads as W
     from oracle import adjoint as OA
