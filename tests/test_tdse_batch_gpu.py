@@ -132,3 +132,29 @@ def test_propagate_one_matches_batch_rows():
                 e = rel_err(got[r], ref[r])
                 print(num_r, n_steps, k, "%.2e" % e, flush=True)
                 assert e < 1e-12, (num_r, n_steps, k)
+
+
+def test_odd_and_large_grids_match_oracle():
+    """Grids whose point count is not a multiple of the 8-point chunk or the 64-chunk tile
+    (odd num_r: Nx even; a partly filled last chunk and tile) and one large grid (Nx = 120001,
+    x in [-6000, 6000]), short runs against the oracle, through the batch and the
+    one-instance paths."""
+    from oracle import ctdse as O
+    rng = np.random.Generator(np.random.PCG64(23))
+    for num_r, dx, n_e in ((2999, 0.1, 160), (1001, 0.2, 200), (4097, 0.1, 120), (120000, 0.1, 40)):
+        # CV at the energy's rounding floor: at 1e-25 the inverse iteration can end in a
+        # 2-cycle of |dE| ~ 1e-15 on such grids, as the reference's own does (tridiag.c:139-142)
+        geo = tdse.Geometry(num_r, dx, 0.25, W.GRID["a"], W.GRID["e_guess"], W.UNEVEN_CV, 2.0)
+        tt = np.arange(n_e) * 0.25
+        f = W.sin2_pulse(tt, 0.1, W.omega_au(800e-9), 1, phase=rng.uniform(0.5, 2.5))
+        dev = torch.tensor(tdse.pad_fields(f), device="cuda")
+        n_steps = n_e + 2
+        x, psi0 = geo.ground_state()
+        ref = O.propagate(num_r, dx, W.GRID["a"], 2.0, 0.25, psi0, x, dev[0].cpu().numpy(), n_steps)
+        batch = tdse.propagate_batch(geo, dev)[0].cpu().numpy()
+        one = tdse.propagate_one(geo, dev[0].contiguous(), n_steps).cpu().numpy()
+        for r, k in enumerate(ROWS[:4]):
+            for name, got in (("batch", batch), ("one", one)):
+                e = rel_err(got[r], ref[k][:n_steps])
+                print(num_r, name, k, "%.2e" % e, flush=True)
+                assert e < TOL, (num_r, name, k)
