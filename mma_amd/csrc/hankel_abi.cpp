@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -125,12 +126,30 @@ int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const 
         (m->include_absorption && (m->n_f2 < 1 || !m->e_f2 || !m->f2)))
         return fail(HK_ERR_ARGUMENT, "missing scattering-factor table");
     hipStream_t s = (hipStream_t)stream;
-    int *flag = nullptr;
-    double *exit_abs = nullptr;  // per frequency: the z-tabulated absorption at the exit plane
-    HK_HIP(hipMalloc(&flag, sizeof(int)));
-    hipError_t e = hipMalloc(&exit_abs, (size_t)no * sizeof(double));
+    // scratch kept between calls (an allocation and a free per call would each synchronise the
+    // device): the failure flag and, per frequency, the z-tabulated absorption at the exit plane
+    static std::mutex scratch_mutex;
+    static int scratch_dev = -1;
+    static int *flag = nullptr;
+    static double *exit_abs = nullptr;
+    static int exit_n = 0;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    int dev = 0;
+    HK_HIP(hipGetDevice(&dev));
+    if (dev != scratch_dev || no > exit_n) {
+        if (flag) (void)hipFree(flag);
+        if (exit_abs) (void)hipFree(exit_abs);
+        flag = nullptr;
+        exit_abs = nullptr;
+        scratch_dev = -1;
+        exit_n = 0;
+        HK_HIP(hipMalloc(&flag, sizeof(int)));
+        HK_HIP(hipMalloc(&exit_abs, (size_t)no * sizeof(double)));
+        scratch_dev = dev;
+        exit_n = no;
+    }
     int zero = 0, got = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, s);
+    hipError_t e = hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, s);
     const int n = m->mode == HK_PRESSURE_ZR ? no * nr : no;
     if (e == hipSuccess) {
         // the scalar branch: one thread per (frequency, plane)
@@ -150,8 +169,6 @@ int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const 
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&got, flag, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(flag);
-    if (exit_abs) (void)hipFree(exit_abs);
     if (e != hipSuccess) return fail(HK_ERR_DEVICE, std::string("pre-factor kernel: ") + hipGetErrorString(e));
     if (got == HK_FLAG_RANGE) return fail(HK_ERR_TABLE, "photon energy outside the tabulated scattering factors");
     if (got == HK_FLAG_F1) return fail(HK_ERR_TABLE, "f1-scattering coefficient not listed for this frequency.");
