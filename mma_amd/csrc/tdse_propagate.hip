@@ -25,8 +25,15 @@
 //
 // Observables (prop.c:293-347) of the state after step k are taken in step k + 1 from the
 // psi it loads anyway (a final pass covers the last state of a partial run); they are
-// reduced per wave, then across waves in LDS in a fixed order (deterministic).  HBM traffic per grid-point-step: psi read + write,
-// 32 bytes; the tables are shared by all instances of a geometry and stay in L2.
+// reduced per wave, then across waves in LDS in a fixed order (deterministic).  HBM traffic
+// per grid-point-step: psi read + write, 32 bytes; the tables are shared by all instances of
+// a geometry and stay in L2.
+//
+// Also here: the two-step pass (tdse_propagate_pair_kernel: segment s of step k kept in
+// registers while segment s-1 of step k+1 is solved from them; the stash form keeps 1/beta in
+// LDS between the two), and the step-parallel form for one instance alone
+// (tdse_parallel_step_kernel: every segment a workgroup of its own, halo on both sides, one
+// launch per step).
 #include "tdse_internal.h"
 #include "tdse_wave.h"
 
