@@ -169,6 +169,25 @@ def test_hankel_long_gridded(contraction, tol):
         assert e < tol, name
 
 
+def test_hankel_long_gridded_nonuniform_r_and_scalar_pressure():
+    """The gridded contraction on a non-uniform radial grid (trapezoid weights of the given
+    r, bins of q = K r / D at arbitrary r) with the scalar-pressure pre-factor, and with an odd
+    plane count, against the FP64 oracle."""
+    from mma_amd import hankel as H
+    rng = np.random.default_rng(17)
+    zall, r, o, S, pressure, rho = _long_case(nz=7, no=20, nr=150, seed=9)
+    r = np.sort(np.concatenate([[0.0], rng.uniform(0, 2e-4, len(r) - 1)]))
+    ref = _oracle_long(zall, r, o, S, 0.7, rho)
+    tgt = H.FSources_provider(zall, r, o, FSource=S)
+    for contraction, tol in (("fp64-grid", 1e-9), ("fp32-mfma", 1e-6)):
+        res = H.Hankel_long(tgt, 1.0, rho, "Ar", 0.7, "Henke", True, "NIST", True, 1.0001, contraction=contraction)
+        for name in ("FF_integrated", "entry_plane_transform", "exit_plane_transform"):
+            e = rel_err(getattr(res, name), ref[name])
+            print(contraction, name, "max %.2e" % e, flush=True)
+            assert e < tol, (contraction, name)
+        tgt = H.FSources_provider(zall, r, o, FSource=S)  # the provider is a single pass
+
+
 def test_gemm_kernels_match_fp64():
     """hk_gemm (FP32 and bf16x3 matrix-core paths) against an FP64 product of the same FP32
     operands, with ragged m and n (tile edges)."""
