@@ -140,6 +140,32 @@ def test_gpu_gradient_matches_oracle(gpu_case):
 
 
 @pytest.mark.gpu
+def test_gpu_gradient_odd_grid_matches_oracle():
+    """The adjoint on a grid whose point count fills neither the last 8-point chunk nor the
+    last 64-chunk tile (num_r = 2999, Nx = 3000), against the oracle gradient."""
+    from mma_amd import adjoint as AD, tdse, workloads as W
+    from oracle import adjoint as OA
+    num_r, dx, n = 2999, 0.1, 300
+    geo = tdse.Geometry(num_r, dx, DT, A_SC, -0.5792, W.UNEVEN_CV, 2.0)
+    x, psi0 = geo.ground_state()
+    fields = np.stack([_field(n, amp=0.06, seed=7), _field(n, amp=0.05, phase=0.4, seed=8)])
+    E = torch.tensor(fields, device="cuda", requires_grad=True)
+    s = AD.TDSEDipole.apply(E, geo, n)
+    gobs = np.random.default_rng(9).standard_normal((len(fields), n))
+    (s * torch.tensor(gobs, device="cuda")).sum().backward()
+    got = E.grad.cpu().numpy()
+    s_np = s.detach().cpu().numpy()
+    for i in range(len(fields)):
+        F, kg = OA.gate(fields[i])
+        ref_s = OA.dipole(num_r, dx, A_SC, 2.0, DT, psi0, x, F, n)
+        ref_g = OA.gradient(num_r, dx, A_SC, 2.0, DT, psi0, x, F, n, gobs[i])
+        ref_g[:kg + 1] = 0.0
+        es, eg = rel_err(s_np[i], ref_s), rel_err(got[i, :n], ref_g)
+        print(i, "odd grid: dipole %.2e grad %.2e" % (es, eg), flush=True)
+        assert es < 1e-8 and eg < 1e-8
+
+
+@pytest.mark.gpu
 def test_gpu_gradient_finite_differences(gpu_case):
     from mma_amd import adjoint as AD
     geo, x, psi0, fields = gpu_case
