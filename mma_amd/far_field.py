@@ -53,6 +53,12 @@ def find_interval(x, x0):
     return k1 - 1
 
 
+def _host(grid):
+    if hasattr(grid, "detach"):  # a torch tensor, possibly on the device
+        grid = grid.detach().cpu().numpy()
+    return np.asarray(grid, dtype=np.float64)
+
+
 def screen(ogrid_au, rgrid_macro, zgrid_macro, h5_handle, omega0_au, harmonic_range, rmax_FF, Nr_FF, distance_FF,
            h5_path=CTDSE_OUTPUTS + "/FSourceTerm", ko_step=1, kr_step=1, kr_max="end", kz_step=1, preset_gas="vacuum",
            pressure=1.0, absorption_tables="Henke", include_absorption=True, dispersion_tables="Henke",
@@ -63,7 +69,7 @@ def screen(ogrid_au, rgrid_macro, zgrid_macro, h5_handle, omega0_au, harmonic_ra
     zgrid_macro: the CTDSE outputs' omegagrid (a.u.), rgrid_coarse and zgrid_coarse; h5_handle:
     anything indexed like the merged file (an h5py file or mma_amd.farm's result).  Returns the
     Hankel_long result with the merged grids (ogrid in SI, the selected window)."""
-    ogrid_au = np.asarray(ogrid_au, dtype=np.float64)
+    ogrid_au, rgrid_macro, zgrid_macro = (_host(g) for g in (ogrid_au, rgrid_macro, zgrid_macro))
     rgrid_FF = np.linspace(0.0, rmax_FF, Nr_FF)
     ko_min = find_interval(ogrid_au / omega0_au, harmonic_range[0])
     ko_max = find_interval(ogrid_au / omega0_au, harmonic_range[-1])

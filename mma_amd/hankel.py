@@ -125,10 +125,41 @@ def trapezoidal_integrator(y, x):
     return np.sum(0.5 * (x[1:] - x[:-1]) * (y[1:] + y[:-1]))
 
 
+def _known_trapezoids():
+    known = [trapezoidal_integrator]
+    for mod, name in ((np, "trapezoid"), (np, "trapz")):
+        if hasattr(mod, name):
+            known.append(getattr(mod, name))
+    try:
+        from scipy import integrate
+        known.append(integrate.trapezoid)
+    except ImportError:
+        pass
+    return known
+
+
+_TRAPEZOID_VERDICT = {}
+
+
 def _is_trapezoid(integrator):
-    if integrator is trapezoidal_integrator:
+    """True for this module's trapezoidal_integrator, numpy's and scipy's trapezoid, and any other
+    callable integrator(y, x) that returns the trapezoid of y over the given x on a probe: complex
+    values on a non-uniform grid (e.g. Hankel_transform.trapezoidal_integrator).  The verdict is
+    kept per callable."""
+    if any(integrator is k for k in _known_trapezoids()):
         return True
-    return getattr(integrator, "__name__", "") in ("trapezoidal_integrator", "trapezoid", "trapz")
+    key = id(integrator)
+    if key not in _TRAPEZOID_VERDICT:
+        x = np.array([0.0, 0.3, 0.35, 1.1, 1.7, 2.0])
+        y = np.array([1.0, -2.0 + 1j, 0.5 - 3j, 4.0, -1.5j, 2.5 + 0.25j])
+        want = np.sum(0.5 * (x[1:] - x[:-1]) * (y[1:] + y[:-1]))
+        try:
+            got = complex(integrator(y, x))
+            ok = abs(got - want) <= 1e-12 * abs(want)
+        except Exception:
+            ok = False
+        _TRAPEZOID_VERDICT[key] = (integrator, ok)  # the callable is held so its id stays unique
+    return _TRAPEZOID_VERDICT[key][1]
 
 
 def _trapezoid_weights(x):
@@ -168,7 +199,10 @@ class FSources_provider:
                 for k1 in range(len(self.zgrid)):
                     re = h5_handle[h5_path][k1 * kz_step, 0:kr_max:kr_step, ko_min:ko_max:ko_step, 0]
                     im = h5_handle[h5_path][k1 * kz_step, 0:kr_max:kr_step, ko_min:ko_max:ko_step, 1]
-                    yield np.squeeze(re + 1j * im).T
+                    if isinstance(re, torch.Tensor):  # a mapping of device tensors (mma_amd.farm.run)
+                        yield torch.complex(re.to(torch.float64), im.to(torch.float64)).squeeze().T
+                    else:
+                        yield np.squeeze(re + 1j * im).T
             self.Fsource_plane = planes()
         else:
             raise ValueError('Wrongly specified input of the class.')
@@ -184,7 +218,11 @@ class FSources_provider:
                 return cast(F[0:nz * kz_step:kz_step, o0:o1:os_, 0:r1:rs].to(device="cuda")).contiguous()
             sel = np.asarray(F)[0:nz * kz_step:kz_step, o0:o1:os_, 0:r1:rs]
             return cast(torch.as_tensor(np.ascontiguousarray(sel), device="cuda"))
-        return torch.stack([cast(torch.as_tensor(np.ascontiguousarray(p), device="cuda")) for p in self.Fsource_plane])
+        def dev(p):
+            if isinstance(p, torch.Tensor):
+                return p.to(device="cuda")
+            return torch.as_tensor(np.ascontiguousarray(p), device="cuda")
+        return torch.stack([cast(dev(p)) for p in self.Fsource_plane]).contiguous()
 
 
 _DEV_TABLES = {}
@@ -198,12 +236,27 @@ def _dev_table(t, key):
     return _DEV_TABLES[(dev, key)]
 
 
+def _host1d(x):
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy().astype(np.float64).ravel()
+    return np.asarray(x, dtype=np.float64).ravel()
+
+
 class _PreFactor:
     """The tables behind get_propagation_pre_factor_function: pre (device, complex
     (Nz, No) or (Nz, No, Nr)), renorm (device (Nz, No) or None)."""
 
     def __init__(self, zgrid, rgrid, ogrid, preset_gas, pressure, absorption_tables, include_absorption,
                  dispersion_tables, include_dispersion, n_ir):
+        if isinstance(pressure, dict) and 'zgrid' in pressure:
+            # the tables are walked in one forward pass: ascending plane grid, strictly ascending
+            # pressure grids (the reference's interp1d and cumulative_trapezoid assume the same)
+            if np.any(np.diff(_host1d(zgrid)) < 0):
+                raise ValueError("zgrid must be ascending for a tabulated pressure")
+            if np.any(np.diff(_host1d(pressure['zgrid'])) <= 0):
+                raise ValueError("pressure['zgrid'] must be strictly ascending")
+            if 'rgrid' in pressure and np.any(np.diff(_host1d(pressure['rgrid'])) <= 0):
+                raise ValueError("pressure['rgrid'] must be strictly ascending")
         zg, rg, og = _dev(zgrid), _dev(rgrid), _dev(ogrid)
         nz, nr, no = len(zg), len(rg), len(og)
         med = hk_medium()

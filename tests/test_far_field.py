@@ -76,3 +76,11 @@ def test_field_to_far_field_matches_oracle():
         assert e < 1e-11, key
     d = FF.outputs(out)
     assert d[FF.OUTPUTS + "/FF_integrated"].shape == (40, k1 - k0, 2)
+    # the farm's own result (device tensors) feeds the driver directly, with the same bits
+    dev_res = farm.run(field, tgrid, r, z, kz_step=1, nz_max=5, kr_step=1, nr_max=6, num_r=1600, dx=0.1, dt=0.25,
+                       a=W.GRID["a"])
+    out_dev = FF.screen(dev_res[P + "/omegagrid"], dev_res[P + "/rgrid_coarse"], dev_res[P + "/zgrid_coarse"],
+                        dev_res, w0, [3.0, 9.0], rmax_FF=4e-3, Nr_FF=40, distance_FF=1.0, include_absorption=False,
+                        include_dispersion=False)
+    for key in ("FF_integrated", "entry_plane_transform", "exit_plane_transform"):
+        np.testing.assert_array_equal(getattr(out_dev, key), getattr(out, key))

@@ -69,6 +69,26 @@ def test_prefactor_tables(case):
         assert ren_f is None
 
 
+def test_prefactor_z_modulated_closed_form():
+    """The device z-modulated pre-factor on the piecewise-linear pressure table with planes
+    before, inside (either side of the kink) and past the table, against its closed form (the
+    same case that pins the oracle's z branch)."""
+    from mma_amd import hankel as H
+    from test_hankel_oracle import RAMP_NIR, RAMP_O, RAMP_Z, RAMP_ZP, _ramp_closed_form, _ramp_p
+    prof = {"zgrid": RAMP_ZP, "value": _ramp_p(RAMP_ZP)}
+    r = np.zeros(2)
+    pre_f, ren_f = H.get_propagation_pre_factor_function(RAMP_Z, r, RAMP_O, "Ar", prof, absorption_tables="Henke",
+                                                         dispersion_tables="NIST",
+                                                         effective_IR_refrective_index=RAMP_NIR)
+    want_pre, want_ren = _ramp_closed_form()
+    got_pre = np.stack([pre_f(k)[0] for k in range(len(RAMP_Z))])
+    got_ren = np.stack([ren_f(k) for k in range(len(RAMP_Z))])
+    e_pre = np.max(np.abs(got_pre - want_pre) / np.max(np.abs(want_pre), axis=0))
+    e_ren = np.max(np.abs(got_ren - want_ren) / want_ren)
+    print("device z-modulated pre-factor vs closed form: %.2e %.2e" % (e_pre, e_ren), flush=True)
+    assert e_pre <= 1e-6 and e_ren <= 1e-6
+
+
 def test_prefactor_errors():
     from mma_amd import hankel as H
     z = np.linspace(0, 1e-3, 3)
@@ -193,16 +213,48 @@ def test_gemm_kernels_match_fp64():
     operands, with ragged m and n (tile edges)."""
     from mma_amd import hankel as H
     rng = np.random.default_rng(7)
-    m, n, k = 200, 330, 256
-    A = torch.tensor(rng.standard_normal((m, k)), dtype=torch.float32, device="cuda")
-    B = torch.tensor(rng.standard_normal((n, k)), dtype=torch.float32, device="cuda")
-    ref = (A.double() @ B.double().T).cpu().numpy()
-    for kind, tol in ((H.HK_GEMM_F32, 1e-6), (H.HK_GEMM_BF16X3, 2e-5), (H.HK_GEMM_BF16X6, 1e-6)):
-        out = torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda")
-        H._check(H._L().hk_gemm(kind, m, n, k, H._ptr(A), k, H._ptr(B), k, H._ptr(out), n, H._stream()))
-        e = rel_err(out.cpu().numpy(), ref)
-        print("gemm kind", kind, "%.2e" % e, flush=True)
-        assert e < tol
+    for m, n, k in ((200, 330, 256), (1024, 600, 5120)):  # the far-field workload contracts K = 5120
+        A = torch.tensor(rng.standard_normal((m, k)), dtype=torch.float32, device="cuda")
+        B = torch.tensor(rng.standard_normal((n, k)), dtype=torch.float32, device="cuda")
+        ref = (A.double() @ B.double().T).cpu().numpy()
+        for kind, tol in ((H.HK_GEMM_F32, 1e-6), (H.HK_GEMM_BF16X3, 2e-5), (H.HK_GEMM_BF16X6, 1e-6)):
+            out = torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda")
+            H._check(H._L().hk_gemm(kind, m, n, k, H._ptr(A), k, H._ptr(B), k, H._ptr(out), n, H._stream()))
+            e = rel_err(out.cpu().numpy(), ref)
+            print("gemm %dx%dx%d kind %d: %.2e" % (m, n, k, kind, e), flush=True)
+            assert e < tol
+
+
+def test_full_size_far_field_error_bound():
+    """The far-field workload at full size (256 planes x 2048 frequencies x 512 radii -> 1024
+    screen radii at 1 m, Ar cell with a z-modulated pressure, complex128 and complex64 sources)
+    through Hankel_long: every gridded contraction against the FP64 direct one ('fp64', pinned to
+    the oracle on the small cases above).  The matrix-core forms must stay within 1e-6 (max |d| /
+    max |ref|) on the integrated field and the entry and exit planes."""
+    from mma_amd import hankel as H, workloads as W
+    zall, r, o, rho, src, distance = W.hankel_workload()
+    pressure = W.hankel_pressure()
+
+    def run(contraction, sources):
+        tgt = H.FSources_provider(zall, r, o, FSource=sources)
+        return H.Hankel_long(tgt, distance, rho, "Ar", pressure, "Henke", True, "NIST", True, 1.0,
+                             store_entry_and_exit_plane_transform=True, contraction=contraction)
+
+    ref = run("fp64", src)
+    names = ("FF_integrated", "entry_plane_transform", "exit_plane_transform")
+    cases = [("fp64-grid", src, 1e-9), ("fp32-mfma", src, 1e-6), ("bf16x3-mfma", src, 1e-6),
+             ("bf16x6-mfma", src, 1e-6)]
+    src64 = src.to(torch.complex64)
+    cases += [("fp32-mfma", src64, 1e-6), ("bf16x3-mfma", src64, 1e-6)]
+    for contraction, sources, tol in cases:
+        res = run(contraction, sources)
+        for name in names:
+            got, want = getattr(res, name), getattr(ref, name)
+            e, e2 = rel_err(got, want), _l2(got, want)
+            print("full size %s %s %s: max %.2e frobenius %.2e" % (contraction, sources.dtype, name, e, e2),
+                  flush=True)
+            assert e <= tol, (contraction, sources.dtype, name, e)
+        del res
 
 
 def test_complex64_sources_and_provider_selection():

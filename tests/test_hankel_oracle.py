@@ -2,8 +2,12 @@
 Hankel pair (with and without the near-field phase), rho = 0, linearity, the longitudinal
 trapezoid over two planes, the provider's plane selection, the scalar pre-factor's closed form
 and renormalisation invariant, the z-modulated branch reducing to the scalar branch for a
-constant pressure, photon conversions, and the high-energy limit of the XUV index (plasma
-dispersion, built from e, eps0 and m instead of the classical electron radius)."""
+constant pressure, the z-modulated branch against its closed form on a piecewise-linear
+pressure table (with planes outside the table) and against three misreadings, the cumulative
+renormalisation against the impulse response of the longitudinal trapezoid, photon
+conversions, and the high-energy limit of the XUV index (plasma dispersion, built from e, eps0
+and m instead of the classical electron radius).  Also the product's input checks that need no
+device: ascending grids for a tabulated pressure, trapezoidal radial integrators only."""
 import math
 
 import numpy as np
@@ -155,3 +159,183 @@ def test_z_and_r_modulated_separable_pressure_matches_z_branch():
 def test_unlisted_f1_raises():
     with pytest.raises(ValueError):
         H.dispersion_function(OMEGA0 * 7, 1.0, "Ar_Henke")  # 10.8 eV: f1 = -9999 in the Henke table
+
+
+# ---------------------------------------------------------------------------------------------
+# z-modulated pre-factor (Hankel_transform.py:146-239) against its closed form on a piecewise
+# linear pressure profile, and the cumulative renormalisation (:663-667) against the impulse
+# response of the longitudinal integral.
+
+EPS0 = 1.0 / (4e-7 * math.pi * H.C_LIGHT ** 2)
+RAMP_ZP = np.linspace(-1e-3, 6e-3, 29)  # pressure table: 0.25 mm nodes starting below z = 0
+RAMP_KINK = RAMP_ZP[16]  # 3 mm, a table node that is not a plane
+# planes: one before the table, table nodes on both sides of the kink (not the kink itself),
+# the table's last node and two planes past it (edge fill)
+RAMP_Z = np.array([-1.5e-3, RAMP_ZP[0], RAMP_ZP[3], RAMP_ZP[7], RAMP_ZP[12], RAMP_ZP[19], RAMP_ZP[28], 6.4e-3,
+                   7e-3])
+RAMP_O = OMEGA0 * np.array([21.0, 33.0, 45.0, 61.0])
+RAMP_NIR = 1.0003
+RAMP_SCALE = 0.25
+
+
+def _ramp_p(z):
+    """Pressure S (0.02 + 12 (z - zp0)) up to the kink, then S (0.068 - 5 (z - z_kink))."""
+    z = np.asarray(z, dtype=np.float64)
+    up = 0.02 + 12.0 * (z - RAMP_ZP[0])
+    pk = 0.02 + 12.0 * (RAMP_KINK - RAMP_ZP[0])
+    return RAMP_SCALE * np.where(z <= RAMP_KINK, up, pk - 5.0 * (z - RAMP_KINK))
+
+
+def _ramp_P(z):
+    """Exact integral of the piecewise-linear pressure from the table's first node to z."""
+    z = np.asarray(z, dtype=np.float64)
+    s = z - RAMP_ZP[0]
+    up = 0.02 * s + 6.0 * s ** 2
+    sk = RAMP_KINK - RAMP_ZP[0]
+    pk = 0.02 + 12.0 * sk
+    d = z - RAMP_KINK
+    down = 0.02 * sk + 6.0 * sk ** 2 + pk * d - 2.5 * d ** 2
+    return RAMP_SCALE * np.where(z <= RAMP_KINK, up, down)
+
+
+def _plasma_coef(omega, f):
+    """N_ref f e^2 / (2 eps0 m omega^2): delta of the XUV index per unit pressure (f = f1) or beta
+    (f = f2), written with e, eps0 and m instead of the classical electron radius and lambda."""
+    return H.N_REF * f * H.ELCHARGE ** 2 / (2.0 * EPS0 * H.ELMASS * omega ** 2)
+
+
+def _ramp_closed_form(dgas="Ar_NIST", agas="Ar_Henke", lost_offset=False, local_beta_p=False,
+                      integrate_on_planes=False, drop_f1=False):
+    """pre(z, w) = p(z) exp(w [i((n_IR - 1)(z - zp0) + A P(z)) + B (P(z) - P(z_last))] / c) and
+    renorm(z, w) = exp(w B (P(z_last) - P(z)) / c), with P = int_{zp0}^{z} p, z clamped to the
+    table (edge fill), A and B the plasma coefficients of f1 and f2.  The keyword switches give
+    the misreadings this closed form has to tell apart from the reference."""
+    zc = np.clip(RAMP_Z, RAMP_ZP[0], RAMP_ZP[-1])
+    P = _ramp_P(zc)
+    if integrate_on_planes:  # pressure interpolated onto the planes, then integrated over them
+        pz = _ramp_p(zc)
+        P = np.concatenate([[0.0], np.cumsum(0.5 * np.diff(zc) * (pz[1:] + pz[:-1]))])
+    pre = np.empty((len(RAMP_Z), len(RAMP_O)), dtype=np.complex128)
+    ren = np.empty((len(RAMP_Z), len(RAMP_O)))
+    for i, w in enumerate(RAMP_O):
+        A = _plasma_coef(w, float(H.getf1(dgas, H.omega_si_to_ev(w))))
+        B = _plasma_coef(w, float(H.getf2(agas, H.omega_si_to_ev(w))))
+        offset = zc if lost_offset else zc - RAMP_ZP[0]
+        phase = (RAMP_NIR - 1.0) * offset + (0.0 if drop_f1 else A) * P
+        absorb = B * (_ramp_p(zc) * (zc - RAMP_ZP[0]) if local_beta_p else P)
+        pre[:, i] = _ramp_p(zc) * np.exp(w * (1j * phase + absorb - absorb[-1]) / H.C_LIGHT)
+        ren[:, i] = np.exp(w * (absorb[-1] - absorb) / H.C_LIGHT)
+    return pre, ren
+
+
+def test_z_modulated_prefactor_closed_form():
+    """The z-modulated branch on a piecewise-linear pressure table, planes on table nodes and
+    outside the table: cumulative_trapezoid and linear interp1d are exact there, so the branch
+    must equal the closed form of _ramp_closed_form.  Phases reach ~400 rad (n_IR) and a few
+    rad (f1), absorption up to ~2.5 nepers; errors are taken per frequency, relative to that
+    frequency's largest value; the tolerance covers the 7e-8 difference between the two unit
+    sets' electron radius."""
+    prof = {"zgrid": RAMP_ZP, "value": _ramp_p(RAMP_ZP)}
+    pre, ren = H.prefactor_tables(RAMP_Z, np.zeros(2), RAMP_O, "Ar", prof, absorption_tables="Henke",
+                                  dispersion_tables="NIST", n_ir=RAMP_NIR)
+    want_pre, want_ren = _ramp_closed_form()
+    e_pre = np.max(np.abs(pre - want_pre) / np.max(np.abs(want_pre), axis=0))
+    e_ren = np.max(np.abs(ren - want_ren) / want_ren)
+    print("z-modulated closed form: pre %.2e renorm %.2e" % (e_pre, e_ren))
+    assert e_pre <= 1e-6 and e_ren <= 1e-6, (e_pre, e_ren)
+    # the case is not degenerate: absorption and the f1 phase term matter at every frequency
+    assert np.all(np.max(want_ren, axis=0) > 1.08)
+    no_f1 = _ramp_closed_form(dgas="Ar_NIST", drop_f1=True)[0]
+    assert np.all(np.max(np.abs(no_f1 - want_pre), axis=0) > 0.5 * np.max(np.abs(want_pre), axis=0))
+
+
+@pytest.mark.parametrize("mistake", ["lost_offset", "local_beta_p", "integrate_on_planes"])
+def test_z_modulated_closed_form_rejects_misreadings(mistake):
+    """Each misreading of the z-modulated branch (zp[0] offset of the n_IR term dropped; the
+    local pressure times the path length for the pressure integral in the absorption; the
+    pressure integrated over the planes after interpolation, which misses the kink between
+    them) moves the tables by far more than the closed-form test's tolerance."""
+    good_pre, good_ren = _ramp_closed_form()
+    bad_pre, bad_ren = _ramp_closed_form(**{mistake: True})
+    e = max(np.max(np.abs(bad_pre - good_pre) / np.max(np.abs(good_pre), axis=0)),
+            np.max(np.abs(bad_ren - good_ren) / good_ren))
+    assert e > 1e-2, (mistake, e)
+
+
+def _impulse_case(j, nz=5, zstep=1e-4):
+    """Planes on a 2-point radial grid [0, R] with the source zero except at plane j, screen
+    radius 0 and no near-field phase: plane k's transform is 1/2 R^2 F_k(w, R) pre_k(w)."""
+    z = np.array([0.0, 1.0, 2.5, 3.0, 4.5])[:nz] * zstep
+    r = np.array([0.0, 2e-5])
+    o = OMEGA0 * np.array([21.0, 29.0])
+    planes = [np.zeros((2, 2), dtype=np.complex128) for _ in range(nz)]
+    planes[j][:, 1] = 2.0 / r[1] ** 2 * np.array([1.0, 0.5 - 0.25j])
+    return z, r, o, planes
+
+
+@pytest.mark.parametrize("j", [0, 2, 4])
+def test_cumulative_renormalisation_impulse_response(j):
+    """Scalar pressure, absorption only: pre_k = p exp(a (z_k - z_last)) and renorm_k =
+    exp(a (z_last - z_k)) with a = w p B / c.  With a source only in plane j, the longitudinal
+    trapezoid through plane k+1 holds plane j with the weight W_k = dz_{j-1}/2 [k + 1 >= j] +
+    dz_j/2 [k >= j], so FF after plane k+1 = W_k p c_j exp(a (z_j - z_last)), and the reference
+    stores cumulative_field[k] = renorm_k times that sum (renorm of plane k, applied after plane
+    k+1 was added, :663-667): cumulative_field[k] = W_k p c_j exp(a (z_j - z_k)).  Applying the
+    renorm of plane k+1 instead, or none, fails."""
+    z, r, o, planes = _impulse_case(j)
+    p = 0.4
+    res = H.hankel_long(z, r, o, planes, 1.0, np.array([0.0]), "Ar", p, "Henke", True, "Henke", False, 1.0,
+                        near_field_factor=False, cumulative=True)
+    cum = res["cumulative_field"][:, 0, :]  # (nz - 1, No)
+    c = np.array([1.0, 0.5 - 0.25j])
+    dz = np.diff(z)
+    a = o * p * _plasma_coef(o, H.getf2("Ar_Henke", H.omega_si_to_ev(o))) / H.C_LIGHT
+    want = np.zeros_like(cum)
+    for k in range(len(z) - 1):
+        wk = (0.5 * dz[j - 1] if 0 < j <= k + 1 else 0.0) + (0.5 * dz[j] if j <= k else 0.0)
+        want[k] = wk * p * c * np.exp(a * (z[j] - z[k]))
+    assert np.max(np.abs(cum - want)) <= 1e-6 * np.max(np.abs(want))
+    wj = 0.5 * ((dz[j - 1] if j > 0 else 0.0) + (dz[j] if j < len(dz) else 0.0))
+    ff_want = wj * p * c * np.exp(a * (z[j] - z[-1]))
+    assert np.max(np.abs(res["FF_integrated"][0] - ff_want)) <= 1e-6 * np.max(np.abs(ff_want))
+    # the absorption over every plane spacing is large enough to tell the orderings apart
+    assert np.min(np.exp(a[:, None] * dz[None, :])) > 1.02
+
+
+def test_tabulated_pressure_needs_ascending_grids():
+    """The product's pre-factor walks the plane grid and the pressure table in one forward
+    pass: descending or repeated grids are refused before anything reaches the device."""
+    from mma_amd import hankel as P
+    prof = {"zgrid": RAMP_ZP, "value": _ramp_p(RAMP_ZP)}
+    r = np.zeros(2)
+    with pytest.raises(ValueError):
+        P.get_propagation_pre_factor_function(RAMP_Z[::-1], r, RAMP_O, "Ar", prof)
+    with pytest.raises(ValueError):
+        P.get_propagation_pre_factor_function(RAMP_Z, r, RAMP_O, "Ar", {"zgrid": RAMP_ZP[::-1],
+                                                                         "value": _ramp_p(RAMP_ZP)})
+    zp = RAMP_ZP.copy()
+    zp[5] = zp[4]
+    with pytest.raises(ValueError):
+        P.get_propagation_pre_factor_function(RAMP_Z, r, RAMP_O, "Ar", {"zgrid": zp, "value": _ramp_p(RAMP_ZP)})
+    with pytest.raises(ValueError):
+        P.get_propagation_pre_factor_function(RAMP_Z, r, RAMP_O, "Ar", {"zgrid": RAMP_ZP, "rgrid": np.array([1.0, 0.5]),
+                                                                         "value": np.ones((len(RAMP_ZP), 2))})
+
+
+def test_only_trapezoidal_integrators_are_accepted():
+    """The product accelerates the trapezoid over the given radii only: numpy's and scipy's
+    trapezoid and any callable that integrates a probe like the trapezoid (the reference's own
+    wrapper) pass; a callable with a trapezoid's name but other semantics is refused."""
+    from scipy import integrate
+    from mma_amd import hankel as P
+
+    def trapezoidal_integrator(y, x):  # the reference's wrapper (Hankel_transform.py:22-27)
+        return integrate.trapezoid(y, x=x)
+
+    def trapz(y, x):  # same name as numpy's, extra weighting
+        return integrate.trapezoid(y * x, x=x)
+
+    for f in (P.trapezoidal_integrator, np.trapezoid, integrate.trapezoid, trapezoidal_integrator):
+        assert P._is_trapezoid(f)
+    for f in (trapz, integrate.simpson, lambda y, x: 0.0):
+        assert not P._is_trapezoid(f)
