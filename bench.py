@@ -62,6 +62,42 @@ class stdout_to_stderr:
         return False
 
 
+def host_info():
+    """The host's CPUs as this process sees them: the machine's count (os.cpu_count, what nproc
+    reports without an affinity mask), this process's affinity mask, the cgroup CPU quota, the
+    OpenMP thread setting and the CPU model; `threads` is what the CPU baselines use: every CPU
+    the affinity mask, the quota and OMP_NUM_THREADS leave to this process."""
+    info = {"cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, p = f.read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(p)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpus"] = quota
+    omp = os.environ.get("OMP_NUM_THREADS")
+    info["omp_num_threads"] = int(omp) if omp and omp.isdigit() else None
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    info["model"] = model
+    threads = info["affinity"]
+    if quota:
+        threads = min(threads, max(1, int(quota)))
+    if info["omp_num_threads"]:
+        threads = min(threads, info["omp_num_threads"])
+    info["threads"] = max(1, threads)
+    return info
+
+
 def cpu_baseline(target_s=8.0, gpu_rows=None):
     """The CPU path on the host cores, on a bounded sample of the same workload: whole
     Gaussian-beam members (all steps, full grid), one member per thread.  When the reference's
@@ -74,7 +110,8 @@ def cpu_baseline(target_s=8.0, gpu_rows=None):
     from concurrent.futures import ThreadPoolExecutor
     from oracle import ctdse as O
     from mma_amd import workloads as W
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    host = host_info()
+    threads = host["threads"]
     g = W.GRID
     nx = g["num_r"] + 1
     t, fields = W.gaussian_beam_batch(n_r=64, n_z=64)
@@ -119,7 +156,7 @@ def cpu_baseline(target_s=8.0, gpu_rows=None):
                   "within_1e-8": bool(np.all(worst <= 1e-8))}
     what = ("reference CTDSE sources (prop.c, tridiag.c, tools.c; gcc -O2 -ffp-contract=off), thread pool"
             if use_ref else "FP64 C oracle (gcc -O2 -ffp-contract=off), OpenMP")
-    return {"value": n_inst * nx * n_out / dt, "unit": "grid-pt-steps/s", "cores": threads,
+    return {"value": n_inst * nx * n_out / dt, "unit": "grid-pt-steps/s", "cores": threads, "host": host,
             "kind": "reference" if use_ref else "port",
             "sample": "%d of the 4096 Gaussian-beam members, %d samples each (Nx=%d), %s, one member per "
                       "thread, %.1f s wall" % (n_inst, n_out, nx, what, dt)}, parity
@@ -235,6 +272,41 @@ def single_measure(geo, field, n_steps, nx, reps=1):
             "launches": n_steps, "form": "ctdse_propagate_one"}
 
 
+def call1dtdse_measure():
+    """The reference-compatible entry points end to end on the single-instance case (Nx 16001,
+    10-cycle 800 nm pulse, 4416 exported samples), host arrays in and out as the reference's
+    ctypes callers use them (TDSE_DLL): Initialise_grid_and_ground_state, a first call1DTDSE
+    (geometry tables built, propagation, spectra, copies to host memory) and a second call on
+    the same inputs (geometry reused).  Wall times in seconds."""
+    from mma_amd import workloads as W
+    from mma_amd.ctdse import TDSE_DLL
+    from mma_amd.ctdse.structures import inputs_def, outputs_def
+    g = W.GRID
+    dll = TDSE_DLL()
+    t, f = W.single_instance()
+    inputs = inputs_def()
+    inputs.init_default_inputs(Eguess=g["e_guess"], num_r=g["num_r"], dx=g["dx"], dt=0.25, trg_a=g["a"], CV=g["cv"],
+                               x_int=g["x_int"])
+    inputs.init_time_and_field(dll, t=t, E=f)
+    t0 = time.perf_counter()
+    dll.init_GS(inputs)
+    t_gs = time.perf_counter() - t0
+    calls = []
+    for _ in range(3):
+        outputs = outputs_def()
+        t0 = time.perf_counter()
+        dll.call1DTDSE(inputs, outputs)
+        calls.append(time.perf_counter() - t0)
+        n_out = outputs.Nt
+        outputs.delete(dll)
+    inputs.delete(dll)
+    log("call1DTDSE: ground state %.1f ms, first call %.1f ms, then %.1f ms" % (t_gs * 1e3, calls[0] * 1e3,
+                                                                              min(calls[1:]) * 1e3))
+    return {"Nx": g["num_r"] + 1, "samples": n_out, "init_ground_state_s": t_gs, "first_call_s": calls[0],
+            "repeat_call_s": min(calls[1:]),
+            "repeat_call_pt_steps_per_s": (g["num_r"] + 1) * n_out / min(calls[1:])}
+
+
 def hankel_measure(cpu=True):
     """Far-field workload: 256 source planes (5 mm Ar cell, z-modulated pressure, NIST f1 /
     Henke f2) x 2048 frequencies (harmonics 11-61 of 800 nm) x 512 radii (0-200 um) onto 1024
@@ -273,10 +345,14 @@ def hankel_measure(cpu=True):
         log("hankel %s: %.2f ms" % (kind, out[kind] * 1e3))
     ref = res["fp64"]
     errors = {}
+    # each output against its own scale: the integrated field (a sum over planes weighted by the
+    # plane spacing) is ~1/Nz of a plane's magnitude, so a max over all three would hide its error
     for kind in ("fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma"):
-        d = res[kind] - ref
-        errors[kind] = {"max_abs_over_max": float(d.abs().max() / ref.abs().max()),
-                        "frobenius": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))}
+        errors[kind] = {}
+        for i, name in enumerate(("FF_integrated", "entry_plane_transform", "exit_plane_transform")):
+            d = res[kind][i] - ref[i]
+            errors[kind][name] = {"max_abs_over_max": float(d.abs().max() / ref[i].abs().max()),
+                                  "frobenius": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref[i]))}
     nq = int(H.grid_size_of(zg, og, rg, rho, distance)[0])
     gemm_flops = 2.0 * len(rho) * (3 * 2 * no) * nq
     info = {
@@ -292,7 +368,7 @@ def hankel_measure(cpu=True):
         "gemm_flops": gemm_flops,
     }
     if cpu:
-        threads = max(1, min(os.cpu_count() or 1, 16))
+        threads = host_info()["threads"]
         tab = H._PreFactor(zg, rg, og, "Ar", pressure, "Henke", True, "NIST", True, 1.0)
         info["cpu_baseline"] = hankel_cpu_baseline(zg, rg, og, rho, src, tab.pre, distance, threads)
     return info
@@ -377,31 +453,46 @@ def stepwise_measure(geo, dev, n_steps, ref):
     return wall, diff
 
 
-def stream_copy_rate(gib, groups_per_cu, reps=5):
-    """Read + write rate (TB/s) of the library's streaming copy (ctdse_stream_copy) over gib GiB,
-    HIP events on the launch stream: the practical ceiling the propagation traffic is compared
-    with."""
+PROBE_KINDS = {"copy": 0, "read": 1, "write": 2}
+
+
+def stream_rate(kind, gib, groups_per_cu, reps=5):
+    """Rate (TB/s of bytes moved) of a bandwidth probe over gib GiB: 'copy' (read + write, both
+    counted), 'read' (read only) or 'write' (write only), from mma_amd/lib/libmmaprobe.so (built
+    with the package, loaded only here); HIP events on the launch stream.  The copy rate is the
+    practical ceiling of the propagation kernel's read + write traffic."""
     import ctypes
     import torch
-    from mma_amd import load_library
-    L = load_library()
-    L.ctdse_stream_copy.restype = ctypes.c_int
-    L.ctdse_stream_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p]
+    from mma_amd import HERE
+    L = ctypes.CDLL(os.path.join(HERE, "lib", "libmmaprobe.so"))
+    L.probe_stream.restype = ctypes.c_int
+    L.probe_stream.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int,
+                               ctypes.c_void_p]
     nbytes = int(gib) << 30
     a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
+    b = torch.zeros_like(a) if kind != "read" else torch.zeros(2, dtype=torch.float64, device="cuda")
+    k = PROBE_KINDS[kind]
     st = torch.cuda.current_stream()
-    assert L.ctdse_stream_copy(a.data_ptr(), b.data_ptr(), nbytes, groups_per_cu, st.cuda_stream) == 0
+
+    def run():
+        code = L.probe_stream(k, a.data_ptr(), b.data_ptr(), nbytes, groups_per_cu, st.cuda_stream)
+        assert code == 0, code
+
+    run()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(st)
     for _ in range(reps):
-        L.ctdse_stream_copy(a.data_ptr(), b.data_ptr(), nbytes, groups_per_cu, st.cuda_stream)
+        run()
     e1.record(st)
     e1.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    assert torch.equal(a, b)
-    rate = 2.0 * nbytes / (ms * 1e-3) / 1e12
-    log("stream copy %d GiB, %d workgroups/CU: %.3f ms, %.3f TB/s read + write" % (gib, groups_per_cu, ms, rate))
+    if kind == "copy":
+        assert torch.equal(a, b)
+    elif kind == "write":
+        assert bool((b == 1.0).all())
+    moved = (2.0 if kind == "copy" else 1.0) * nbytes
+    rate = moved / (ms * 1e-3) / 1e12
+    log("stream %s %d GiB, %d workgroups/CU: %.3f ms, %.3f TB/s" % (kind, gib, groups_per_cu, ms, rate))
     del a, b
     return rate
 
@@ -429,13 +520,13 @@ def main():
     ap.add_argument("--no-hankel", action="store_true")
     ap.add_argument("--only-hankel", action="store_true", help="time only the far-field workload")
     ap.add_argument("--stream-probe", type=int, default=0, metavar="GIB",
-                    help="time the library's streaming copy of GIB GiB (read + write ceiling) and exit")
+                    help="time the copy, read-only and write-only bandwidth probes over GIB GiB and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB three times and exit (HBM counter calibration)")
     args = ap.parse_args()
     if args.stream_probe:
-        res = {g: stream_copy_rate(args.stream_probe, g) for g in (4, 8, 16, 32)}
-        print(json.dumps({"stream_copy_TBps": res, "GiB": args.stream_probe}), flush=True)
+        res = {kind: {g: stream_rate(kind, args.stream_probe, g) for g in (4, 8, 16, 32)} for kind in PROBE_KINDS}
+        print(json.dumps({"stream_TBps": res, "GiB": args.stream_probe}), flush=True)
         return
     if args.copy_probe:
         import torch
@@ -461,8 +552,10 @@ def main():
         geo = tdse.Geometry(g["num_r"], g["dx"], 0.25, g["a"], g["e_guess"], g["cv"], g["x_int"])
         t, fields = W.gaussian_beam_batch(n_r=1, n_z=1)
         dev = torch.tensor(tdse.pad_fields(fields), device="cuda")
-        print(json.dumps({"single": single_measure(geo, dev[0], fields.shape[1] + 2, g["num_r"] + 1, reps=3)}),
-              flush=True)
+        one = single_measure(geo, dev[0], fields.shape[1] + 2, g["num_r"] + 1, reps=3)
+        with stdout_to_stderr():
+            dropin = call1dtdse_measure()
+        print(json.dumps({"single": one, "call1dtdse": dropin}), flush=True)
         return
     if args.only_adjoint:
         import torch
@@ -524,9 +617,11 @@ def main():
         log("stepwise: %.3f s" % sw_wall)
     plan.close()
 
-    single = None
+    single, dropin = None, None
     if not args.no_single:
         single = single_measure(geo, dev[0], n_steps, nx)["pt_steps_per_s"]
+        with stdout_to_stderr():
+            dropin = call1dtdse_measure()
 
     uneven = None
     if not args.no_uneven:
@@ -547,7 +642,9 @@ def main():
         hk = hankel_measure(cpu=not args.no_cpu)
         torch.cuda.empty_cache()
 
-    copy_gbs = 1e3 * stream_copy_rate(16, 32)  # the box's read + write ceiling, same units as achieved
+    # the box's practical ceilings, same units as achieved: read + write (the propagation's
+    # traffic mix), read only and write only
+    copy_gbs, read_gbs, write_gbs = (1e3 * stream_rate(kind, 16, 32) for kind in ("copy", "read", "write"))
     torch.cuda.empty_cache()
 
     cpu, parity = None, None
@@ -566,8 +663,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "copy_ceiling": copy_gbs, "frac_of_copy_ceiling": achieved / copy_gbs if copy_gbs else None,
+                     "read_only_ceiling": read_gbs, "write_only_ceiling": write_gbs,
                      "traffic": MEASURED_TRAFFIC_BYTES if (args.n_r, args.n_z) == (64, 64) else None},
         "cpu_baseline": cpu,
+        "gpu_over_cpu_baseline": (value / cpu["value"]) if cpu else None,
         "parity_vs_reference": parity,
         "launch": {"form": "hipGraph replay of one persistent launch", "config": list(tuned.config.as_tuple()),
                    "tune_table": [list(r) for r in tuned.table[:6]], "tune_s": t_tune, "cached": tuned.cached},
@@ -575,6 +674,7 @@ def main():
         "persistent_over_stepwise": (value / stepwise["pt_steps_per_s"]) if stepwise else None,
         "uneven": uneven,
         "per_instance_pt_steps_per_s": single,
+        "call1dtdse": dropin,
         "batched_over_per_instance": (value / single) if single else None,
         "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,
         "adjoint": adj,

@@ -264,13 +264,6 @@ int ctdse_batch_plan_graph_launch(ctdse_batch_plan *plan, void *stream);
 int ctdse_batch_plan_info(const ctdse_batch_plan *plan, ctdse_launch_config *config, int *workgroups);
 void ctdse_batch_plan_destroy(ctdse_batch_plan *plan);
 
-/* Measurement helper (no counterpart in the reference): queues a streaming copy of n_bytes
- * (a multiple of 16) from src to dst, device memory, on `stream` with groups_per_cu
- * workgroups of 256 lanes per compute unit; returns without waiting.  The read-and-write
- * rate it reaches is the practical ceiling the propagation kernel's traffic is compared
- * with. */
-int ctdse_stream_copy(const double *src, double *dst, long long n_bytes, int groups_per_cu, void *stream);
-
 /* Length, in complex doubles, of one device state vector of this geometry (the tiled
  * layout with its zero padding); states below are arrays of such vectors. */
 long long ctdse_state_length(const ctdse_geometry *g);

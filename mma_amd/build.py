@@ -1,5 +1,6 @@
-"""Builds the native libraries for gfx950 with hipcc: mma_amd/lib/libsingleTDSE.so (TDSE)
-and mma_amd/lib/libmmahankel.so (Hankel far-field transform).
+"""Builds the native libraries for gfx950 with hipcc: mma_amd/lib/libsingleTDSE.so (TDSE),
+mma_amd/lib/libmmahankel.so (Hankel far-field transform) and mma_amd/lib/libmmaprobe.so (the
+benchmark's bandwidth probes, loaded by bench.py only).
 
 Usage: python -m mma_amd.build   (or mma_amd.build.build() from Python)
 """
@@ -33,12 +34,15 @@ SOURCES = [
     ("hankel_contract.hip", []),
     ("hankel_gemm.hip", []),
     ("hankel_abi.cpp", ["-x", "hip"]),
+    ("stream_probe.hip", []),
 ]
 
 LIBS = {
     "libsingleTDSE.so": ["tdse_setup.hip", "tdse_analysis.hip", "tdse_propagate.hip", "tdse_adjoint.hip",
                          "tdse_reverse.hip", "ctdse_abi.cpp"],
     "libmmahankel.so": ["hankel_medium.hip", "hankel_contract.hip", "hankel_gemm.hip", "hankel_abi.cpp"],
+    # bandwidth probes for bench.py only (not part of the product)
+    "libmmaprobe.so": ["stream_probe.hip"],
 }
 
 

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -138,28 +139,6 @@ int gs_status(const GsJob &J, const std::string &who)
     return CTDSE_OK;
 }
 
-int run_ground_state(int num_r, double dx, double a, double e_guess, double cv, DevMem &x, DevMem &psi0, double *e_out,
-                     int *iters)
-{
-    const long long nx = num_r + 1;
-    HIP_CHECK(x.alloc(nx * sizeof(double)));
-    HIP_CHECK(psi0.alloc(2 * nx * sizeof(double)));
-    std::vector<GsJob> jobs(1);
-    GsJob &J = jobs[0];
-    J.num_r = num_r;
-    J.dx = dx;
-    J.a = a;
-    J.e_guess = e_guess;
-    J.cv = cv;
-    J.x = x.as<double>();
-    J.psi0 = psi0.as<double>();
-    const int st = run_gs_jobs(jobs);
-    if (st != CTDSE_OK) return st;
-    *e_out = J.energy;
-    *iters = J.iters;
-    return gs_status(J, "");
-}
-
 TableJob table_job(ctdse_geometry *G)
 {
     TableJob J{};
@@ -238,11 +217,56 @@ int finish_tables(ctdse_geometry *G, const TableJob &J, const std::string &who)
     return CTDSE_OK;
 }
 
+// Host copies of one geometry's grid and ground state (natural order, interleaved).
+struct HostGrid {
+    std::vector<double> x, psi0;
+};
+
 // Builds the factor tables of several geometries (x and psi0_nat set) in grouped launches;
 // note(index[k], code) is called (with the message set) for each geometry that fails.
+// host: one geometry alone with its grid and ground state on the host; its serial pivot scan
+// then runs on a host core (a few ms for 16001 points, against ~24 ms in one GPU lane), with
+// the same bits.
 template <class Note>
-int build_tables(const std::vector<ctdse_geometry *> &gs, const std::vector<int> &index, Note note)
+int build_tables(const std::vector<ctdse_geometry *> &gs, const std::vector<int> &index, Note note,
+                 const HostGrid *host = nullptr)
 {
+    if (host && gs.size() == 1) {
+        ctdse_geometry *G = gs[0];
+        const size_t nx = G->num_r + 1;
+        DevMem piv, d_job;
+        if (alloc_tables(G) != CTDSE_OK) return g_status;
+        HIP_CHECK(piv.alloc(2 * nx * sizeof(double)));
+        HIP_CHECK(d_job.alloc(sizeof(TableJob)));
+        std::vector<double> piv_h(2 * nx);
+        TableJob Jh = table_job(G);
+        Jh.x = host->x.data();
+        Jh.psi0_nat = host->psi0.data();
+        Jh.pivot_work = piv_h.data();
+        tdse_tables_pivot_host(Jh);
+        TableJob J = table_job(G);
+        J.pivot_work = piv.as<double>();
+        J.rho_fwd = Jh.rho_fwd;
+        J.rho_bwd = Jh.rho_bwd;
+        J.pint_lo = Jh.pint_lo;
+        J.pint_hi = Jh.pint_hi;
+        J.p0_lo = Jh.p0_lo;
+        J.p0_hi = Jh.p0_hi;
+        J.pint0 = Jh.pint0;
+        HIP_CHECK(hipMemcpy(piv.p, piv_h.data(), 2 * nx * sizeof(double), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(d_job.p, &J, sizeof(TableJob), hipMemcpyHostToDevice));
+        const int nchunks = (int)((nx + TDSE_CHUNK - 1) / TDSE_CHUNK);
+        hipLaunchKernelGGL(tdse_tables_grid_kernel, dim3(64, 1), dim3(256), 0, 0, d_job.as<TableJob>());
+        hipLaunchKernelGGL(tdse_tables_ib_kernel, dim3((unsigned)((nx + 255) / 256), 1), dim3(256), 0, 0,
+                           d_job.as<TableJob>());
+        hipLaunchKernelGGL(tdse_tables_chunk_kernel, dim3((nchunks + 255) / 256, 1), dim3(256), 0, 0,
+                           d_job.as<TableJob>());
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(nullptr));  // the scratch buffers are freed on return
+        const int st = finish_tables(G, J, "");
+        if (st != CTDSE_OK) note(index[0], st);
+        return CTDSE_OK;
+    }
     size_t i0 = 0;
     while (i0 < gs.size()) {
         size_t i1 = i0, bytes = 0;
@@ -270,6 +294,8 @@ int build_tables(const std::vector<ctdse_geometry *> &gs, const std::vector<int>
         HIP_CHECK(hipMemcpy(d_jobs.p, jobs.data(), m * sizeof(TableJob), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(tdse_tables_grid_kernel, dim3(64, m), dim3(256), 0, 0, d_jobs.as<TableJob>());
         hipLaunchKernelGGL(tdse_tables_pivot_kernel, dim3(m), dim3(64), 0, 0, d_jobs.as<TableJob>());
+        hipLaunchKernelGGL(tdse_tables_ib_kernel, dim3((unsigned)((max_chunks * TDSE_CHUNK + 255) / 256), m), dim3(256),
+                           0, 0, d_jobs.as<TableJob>());
         hipLaunchKernelGGL(tdse_tables_chunk_kernel, dim3((max_chunks + 255) / 256, m), dim3(256), 0, 0,
                            d_jobs.as<TableJob>());
         HIP_CHECK(hipGetLastError());
@@ -301,6 +327,53 @@ int check_spec(const GeomSpec &g, const std::string &who)
     return CTDSE_OK;
 }
 
+// Grid and ground state of one geometry on the host: x_k = k dx - num_r dx / 2 (tools.c:62-79)
+// unless given, psi0 given or by the inverse iteration (tdse_ground_state_host, the same bits
+// as the reference).  Sets G's energy and iteration count; CTDSE_ERR_GS_NOT_CONVERGED still
+// leaves the last iterate in hg.
+int host_grid(const GeomSpec &g, ctdse_geometry *G, HostGrid &hg)
+{
+    const size_t nx = g.num_r + 1;
+    hg.x.resize(nx);
+    hg.psi0.resize(2 * nx);
+    if (g.x_host) {
+        std::copy(g.x_host, g.x_host + nx, hg.x.begin());
+    } else {
+        const double half_span = 0.5 * g.num_r * g.dx;
+        for (size_t k = 0; k < nx; k++) hg.x[k] = (double)k * g.dx - half_span;
+    }
+    if (g.psi0_host) {
+        std::copy(g.psi0_host, g.psi0_host + 2 * nx, hg.psi0.begin());
+        return CTDSE_OK;
+    }
+    std::vector<double> work(16 * ((size_t)g.num_r + 2));
+    GsJob J{};
+    J.num_r = g.num_r;
+    J.max_iter = kGsMaxIter;
+    J.dx = g.dx;
+    J.a = g.a;
+    J.e_guess = g.e_guess;
+    J.cv = g.cv;
+    J.x = hg.x.data();  // the iteration writes the same grid
+    J.psi0 = hg.psi0.data();
+    J.work = work.data();
+    tdse_ground_state_host(J);
+    G->e_init = J.energy;
+    G->gs_iters = J.iters;
+    return gs_status(J, "");
+}
+
+int upload_grid(ctdse_geometry *G, const HostGrid &hg, const std::string &who)
+{
+    hipError_t e = G->x.alloc(hg.x.size() * sizeof(double));
+    if (e == hipSuccess) e = G->psi0_nat.alloc(hg.psi0.size() * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(G->x.p, hg.x.data(), hg.x.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(G->psi0_nat.p, hg.psi0.data(), hg.psi0.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_status(CTDSE_ERR_DEVICE, who + "geometry upload: " + hipGetErrorString(e));
+    return CTDSE_OK;
+}
+
 // Builds geometries: ground states of all that need one in grouped launches, then the
 // tables.  out[i] is null where status[i] is a hard failure; a ground state that did not
 // converge (CTDSE_ERR_GS_NOT_CONVERGED) still yields a geometry, as in the reference, which
@@ -327,6 +400,10 @@ int geometries_from(const std::vector<GeomSpec> &spec, std::vector<ctdse_geometr
     auto who = [&](int i) { return n > 1 ? "geometry " + std::to_string(i) + ": " : std::string(); };
     std::vector<GsJob> jobs;
     std::vector<int> job_of;
+    // one geometry alone: its serial steps (ground state, pivots) run on the host, where one
+    // core is far faster than one GPU lane; many geometries run them side by side on the GPU
+    const bool on_host = n == 1;
+    HostGrid hg;
     for (int i = 0; i < n; i++) {
         const GeomSpec &g = spec[i];
         if (check_spec(g, who(i)) != CTDSE_OK) {
@@ -341,6 +418,14 @@ int geometries_from(const std::vector<GeomSpec> &spec, std::vector<ctdse_geometr
         G->a = g.a;
         G->x_int = g.x_int;
         const size_t nx = g.num_r + 1;
+        if (on_host) {
+            const int gst = host_grid(g, G, hg);
+            if (gst != CTDSE_OK) note(i, gst);
+            if (gst != CTDSE_OK && gst != CTDSE_ERR_GS_NOT_CONVERGED) continue;
+            const int st = upload_grid(G, hg, who(i));
+            if (st != CTDSE_OK) note(i, st);
+            continue;
+        }
         hipError_t e = G->x.alloc(nx * sizeof(double));
         if (e == hipSuccess) e = G->psi0_nat.alloc(2 * nx * sizeof(double));
         if (e == hipSuccess && g.psi0_host) {
@@ -397,7 +482,7 @@ int geometries_from(const std::vector<GeomSpec> &spec, std::vector<ctdse_geometr
             tab.push_back(out[i]);
             tab_index.push_back(i);
         }
-    if (build_tables(tab, tab_index, note) != CTDSE_OK) {
+    if (build_tables(tab, tab_index, note, on_host ? &hg : nullptr) != CTDSE_OK) {
         for (int i = 0; i < n; i++) {
             delete out[i];
             out[i] = nullptr;
@@ -427,6 +512,51 @@ int geometry_from(int num_r, double dx, double dt, double a, double e_guess, dou
     const int code = geometries_from(spec, gs, st);
     *out = gs[0];
     return code;
+}
+
+// call1DTDSE's geometry cache: the last geometry it built, reused while the calls' grid,
+// time step, atom, PopInt window, ground state and x grid are the same (a task farm of calls
+// shares one ground state, MPI-TDSE.c:145), so a call after the first builds no tables.
+// Held through shared_ptr: a call keeps its geometry alive while another call replaces the
+// entry.  Never destroyed (device memory is not released from a static destructor).
+struct GeomCache {
+    std::mutex m;
+    std::shared_ptr<ctdse_geometry> geom;
+    int device = -1, num_r = 0;
+    double dx = 0, dt = 0, a = 0, x_int = 0;
+    std::vector<double> psi0, x;
+};
+GeomCache *g_geom_cache = new GeomCache;
+
+std::shared_ptr<ctdse_geometry> cached_geometry(const inputs_def &in, double dt, int *st)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+    const size_t nx = in.num_r + 1;
+    GeomCache &C = *g_geom_cache;
+    std::lock_guard<std::mutex> lock(C.m);
+    if (C.geom && C.device == dev && C.num_r == in.num_r && C.dx == in.dx && C.dt == dt && C.a == in.trg.a &&
+        C.x_int == in.x_int && std::memcmp(C.psi0.data(), in.psi0, 2 * nx * sizeof(double)) == 0 &&
+        std::memcmp(C.x.data(), in.x, nx * sizeof(double)) == 0) {
+        *st = CTDSE_OK;
+        return C.geom;
+    }
+    ctdse_geometry *G = nullptr;
+    *st = geometry_from(in.num_r, in.dx, dt, in.trg.a, in.Eguess, in.CV, in.x_int, in.psi0, in.x, &G);
+    if (*st != CTDSE_OK) {
+        delete G;
+        return nullptr;
+    }
+    C.geom.reset(G);
+    C.device = dev;
+    C.num_r = in.num_r;
+    C.dx = in.dx;
+    C.dt = dt;
+    C.a = in.trg.a;
+    C.x_int = in.x_int;
+    C.psi0.assign(in.psi0, in.psi0 + 2 * nx);
+    C.x.assign(in.x, in.x + nx);
+    return C.geom;
 }
 
 struct InstSpec {
@@ -675,12 +805,20 @@ int par_waves(int halo_lanes)
     return nw * 64 - 2 * halo_lanes >= 64 ? nw : 0;
 }
 
-// One instance over its whole run with no partial runs or saved states: the segments of a
-// step are solved side by side (tdse_parallel_step_kernel, psi double-buffered), so a step
-// costs one segment's latency instead of nseg of them; snapshots and the final state are
-// copied out between steps.  Returns CTDSE_ERR_UNSUPPORTED (status untouched) when the
+// Descriptors of a step-parallel run written on the device from kernel arguments, so the
+// run needs no host-to-device copy (and no staging memory).
+__global__ void tdse_put_desc_kernel(TdseGeom g, TdseInst inst, TdseGeom *g_out, TdseInst *inst_out)
+{
+    if (threadIdx.x == 0) {
+        *g_out = g;
+        *inst_out = inst;
+    }
+}
+
+// Launch shape of a step-parallel run of one instance (propagate_steps): waves per
+// workgroup and segments per step.  Returns CTDSE_ERR_UNSUPPORTED (status untouched) when the
 // descriptor needs the persistent path.
-int propagate_steps(const InstSpec &s, hipStream_t stream)
+int steps_shape(const InstSpec &s, int *nw_out, int *nseg_out)
 {
     if (!s.geom || s.n_run != 0 || s.k0 != 0 || s.psi_init || s.save || s.pregated || !s.out || s.n_steps < 1)
         return CTDSE_ERR_UNSUPPORTED;
@@ -688,53 +826,89 @@ int propagate_steps(const InstSpec &s, hipStream_t stream)
     const int nw = par_waves(g.halo_lanes);
     if (nw == 0) return CTDSE_ERR_UNSUPPORTED;
     const int own = nw * 64 - 2 * g.halo_lanes;
-    const int nseg = (g.nchunks + own - 1) / own;
+    *nw_out = nw;
+    *nseg_out = (g.nchunks + own - 1) / own;
+    return CTDSE_OK;
+}
+
+// Device buffers of a step-parallel run: the two descriptors, per-segment partial sums of
+// the observables (2 x nseg x 5 doubles), the gated field (n_steps + 1 doubles) and two
+// state vectors (npad complex doubles each).
+struct StepBufs {
+    TdseGeom *geom;
+    TdseInst *inst;
+    double *partial;
+    double *gated;
+    double2 *psi[2];
+};
+
+// Queues a step-parallel run into the given buffers: no allocation and no wait.  Every
+// buffer is (re)initialised here, so buffers left over from an earlier run may be reused.
+int enqueue_steps(const InstSpec &s, int nw, int nseg, const StepBufs &b, hipStream_t stream)
+{
+    const TdseGeom &g = s.geom->dev;
     BatchHost B;
     if (describe({s}, B) != CTDSE_OK) return g_status;
     const int n_field = s.n_steps + 1;
-    DevMem states, partial, gated, d_geom, d_inst;
-    HIP_CHECK(states.alloc(2 * (size_t)g.npad * sizeof(double2)));
-    HIP_CHECK(partial.alloc(2 * (size_t)nseg * 5 * sizeof(double)));
-    HIP_CHECK(gated.alloc((size_t)n_field * sizeof(double)));
-    HIP_CHECK(d_geom.alloc(sizeof(TdseGeom)));
-    HIP_CHECK(d_inst.alloc(sizeof(TdseInst)));
-    B.insts[0].field = gated.as<double>();
-    B.insts[0].pregated = 1;
-    double2 *psi[2] = {states.as<double2>(), states.as<double2>() + g.npad};
-    std::lock_guard<std::mutex> lock(g_launch_mutex);
-    HIP_CHECK(hipMemcpyAsync(d_geom.p, &g, sizeof(TdseGeom), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(d_inst.p, B.insts.data(), sizeof(TdseInst), hipMemcpyHostToDevice, stream));
-    // psi0 into the first buffer; the second's zero tile and padding rows stay zero (the
-    // steps write own chunks only)
-    HIP_CHECK(hipMemcpyAsync(psi[0], g.psi0, (size_t)g.npad * sizeof(double2), hipMemcpyDeviceToDevice, stream));
-    HIP_CHECK(hipMemsetAsync(psi[1], 0, (size_t)g.npad * sizeof(double2), stream));
-    hipLaunchKernelGGL(tdse_gate_kernel, dim3(1), dim3(256), 0, stream, s.field, n_field, gated.as<double>());
+    TdseInst inst = B.insts[0];
+    inst.field = b.gated;
+    inst.pregated = 1;
+    hipLaunchKernelGGL(tdse_put_desc_kernel, dim3(1), dim3(64), 0, stream, g, inst, b.geom, b.inst);
+    // psi0 into the first buffer; the second one zeroed (by a kernel, not a memset node), its
+    // zero tiles and padding rows stay zero (the steps write own chunks only)
+    HIP_CHECK(hipMemcpyAsync(b.psi[0], g.psi0, (size_t)g.npad * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+    const long long n_int = 4 * g.npad;
+    hipLaunchKernelGGL(tdse_fill_int_kernel, dim3((unsigned)((n_int + 255) / 256)), dim3(256), 0, stream,
+                       reinterpret_cast<int *>(b.psi[1]), (int)n_int, 0);
+    hipLaunchKernelGGL(tdse_gate_kernel, dim3(1), dim3(256), 0, stream, s.field, n_field, b.gated);
     const dim3 untile_grid((g.nx + 255) / 256);
     const int n_snap = s.snap ? s.n_snap : 0, every = std::max(1, s.snap_every);
     int i_snap = 0;
     for (int k = 0; k < s.n_steps; k++) {
-        hipLaunchKernelGGL(tdse_parallel_step_kernel, dim3(nseg, 1), dim3(64 * nw), 0, stream,
-                           d_geom.as<TdseGeom>(), d_inst.as<TdseInst>(), k, psi[k & 1], psi[(k + 1) & 1],
-                           (long long)g.npad, partial.as<double>(), nseg);
+        hipLaunchKernelGGL(tdse_parallel_step_kernel, dim3(nseg, 1), dim3(64 * nw), 0, stream, b.geom, b.inst, k,
+                           b.psi[k & 1], b.psi[(k + 1) & 1], (long long)g.npad, b.partial, nseg);
         // prop.c:248-255: row i_snap gets psi after step k when k % every == 0
         if (k % every == 0 && i_snap < n_snap - 1)
-            hipLaunchKernelGGL(tdse_untile_kernel, untile_grid, dim3(256), 0, stream, psi[(k + 1) & 1], g.nx,
+            hipLaunchKernelGGL(tdse_untile_kernel, untile_grid, dim3(256), 0, stream, b.psi[(k + 1) & 1], g.nx,
                                s.snap + (size_t)i_snap++ * 2 * g.nx);
     }
     // prop.c:256-262: after the last step, the last row gets the final state if every earlier
     // row was used; the final state itself is optional output
-    const double2 *fin = psi[s.n_steps & 1];
+    const double2 *fin = b.psi[s.n_steps & 1];
     if (n_snap > 0 && i_snap == n_snap - 1)
         hipLaunchKernelGGL(tdse_untile_kernel, untile_grid, dim3(256), 0, stream, fin, g.nx,
                            s.snap + (size_t)(n_snap - 1) * 2 * g.nx);
     if (s.psi_final) hipLaunchKernelGGL(tdse_untile_kernel, untile_grid, dim3(256), 0, stream, fin, g.nx, s.psi_final);
-    hipLaunchKernelGGL(tdse_parallel_finish_kernel, dim3(1), dim3(64), 0, stream, d_inst.as<TdseInst>(), 1,
-                       partial.as<double>(), nseg);
-    // wait before reporting a launch error: the buffers freed on return may still be in use
-    const hipError_t launched = hipGetLastError();
-    HIP_CHECK(hipStreamSynchronize(stream));
-    HIP_CHECK(launched);
+    hipLaunchKernelGGL(tdse_parallel_finish_kernel, dim3(1), dim3(64), 0, stream, b.inst, 1, b.partial, nseg);
+    HIP_CHECK(hipGetLastError());
     return CTDSE_OK;
+}
+
+// One instance over its whole run with no partial runs or saved states: the segments of a
+// step are solved side by side (tdse_parallel_step_kernel, psi double-buffered), so a step
+// costs one segment's latency instead of nseg of them; snapshots and the final state are
+// copied out between steps.  Returns CTDSE_ERR_UNSUPPORTED (status untouched) when the
+// descriptor needs the persistent path.
+int propagate_steps(const InstSpec &s, hipStream_t stream)
+{
+    int nw = 0, nseg = 0;
+    const int shaped = steps_shape(s, &nw, &nseg);
+    if (shaped != CTDSE_OK) return shaped;
+    const TdseGeom &g = s.geom->dev;
+    DevMem states, partial, gated, d_geom, d_inst;
+    HIP_CHECK(states.alloc(2 * (size_t)g.npad * sizeof(double2)));
+    HIP_CHECK(partial.alloc(2 * (size_t)nseg * 5 * sizeof(double)));
+    HIP_CHECK(gated.alloc((size_t)(s.n_steps + 1) * sizeof(double)));
+    HIP_CHECK(d_geom.alloc(sizeof(TdseGeom)));
+    HIP_CHECK(d_inst.alloc(sizeof(TdseInst)));
+    const StepBufs b{d_geom.as<TdseGeom>(), d_inst.as<TdseInst>(), partial.as<double>(), gated.as<double>(),
+                     {states.as<double2>(), states.as<double2>() + g.npad}};
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    const int queued = enqueue_steps(s, nw, nseg, b, stream);
+    // wait before returning (also after a launch error): the buffers freed on return may
+    // still be in use
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return queued;
 }
 
 // Plan path: everything allocated and uploaded here, once.
@@ -894,6 +1068,8 @@ int spectrum_to_host(const double *sig_dev, int n, double dt, double tmax, doubl
     if (omega) *omega = host_copy(om.as<double>(), nc, nc);
     *F = host_copy(f.as<double>(), 2 * nc, 2 * nc);
     *M2 = host_copy(m2.as<double>(), nc, nc);
+    if ((omega && !*omega) || !*F || !*M2)
+        return set_status(CTDSE_ERR_DEVICE, "spectrum: allocating or copying the host arrays failed");
     return CTDSE_OK;
 }
 
@@ -1085,21 +1261,6 @@ int ctdse_batch_plan_info(const ctdse_batch_plan *plan, ctdse_launch_config *con
 
 void ctdse_batch_plan_destroy(ctdse_batch_plan *plan) { delete plan; }
 
-int ctdse_stream_copy(const double *src, double *dst, long long n_bytes, int groups_per_cu, void *stream)
-{
-    clear_status();
-    if (!src || !dst || n_bytes < 16 || n_bytes % 16 != 0 || groups_per_cu < 1 || groups_per_cu > 64)
-        return set_status(CTDSE_ERR_ARGUMENT, "stream copy: n_bytes a positive multiple of 16, 1..64 groups per CU");
-    if (device_ready() != CTDSE_OK) return g_status;
-    int dev = 0, cus = 0;
-    HIP_CHECK(hipGetDevice(&dev));
-    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    hipLaunchKernelGGL(tdse_stream_copy_kernel, dim3(cus * groups_per_cu), dim3(256), 0, (hipStream_t)stream,
-                       (const double2 *)src, (double2 *)dst, n_bytes / 16);
-    HIP_CHECK(hipGetLastError());
-    return CTDSE_OK;
-}
-
 long long ctdse_state_length(const ctdse_geometry *g)
 {
     clear_status();
@@ -1206,17 +1367,39 @@ void Initialise_grid_and_ground_state(inputs_def *in)
         set_status(CTDSE_ERR_ARGUMENT, "num_r >= 2, dx > 0 and trg.a > 0 are required");
         return;
     }
-    if (device_ready() != CTDSE_OK) return;
-    DevMem x, psi0;
-    double e = 0;
-    int it = 0;
-    const int st = run_ground_state(in->num_r, in->dx, in->trg.a, in->Eguess, in->CV, x, psi0, &e, &it);
-    if (st != CTDSE_OK && st != CTDSE_ERR_GS_NOT_CONVERGED && st != CTDSE_ERR_ZERO_PIVOT) return;
+    // one geometry's inverse iteration is a serial chain: it runs on the host, with the
+    // reference's operation order (the same bits as tools.c:31-51 and the GPU lanes of
+    // ctdse_geometry_create_batch)
     const size_t nx = in->num_r + 1;
-    in->x = host_copy(x.as<double>(), nx, nx);
-    in->psi0 = host_copy(psi0.as<double>(), 2 * nx, 2 * nx);
-    in->Einit = e;
-    if (!in->x || !in->psi0) set_status(CTDSE_ERR_DEVICE, "ground state download failed");
+    double *x = (double *)calloc(nx, sizeof(double));
+    double *psi0 = (double *)calloc(2 * nx, sizeof(double));
+    std::vector<double> work;
+    try {
+        work.resize(16 * (nx + 1));
+    } catch (const std::bad_alloc &) {
+        work.clear();
+    }
+    if (!x || !psi0 || work.empty()) {
+        free(x);
+        free(psi0);
+        set_status(CTDSE_ERR_ARGUMENT, "Initialise_grid_and_ground_state: host allocation failed");
+        return;
+    }
+    GsJob J{};
+    J.num_r = in->num_r;
+    J.max_iter = kGsMaxIter;
+    J.dx = in->dx;
+    J.a = in->trg.a;
+    J.e_guess = in->Eguess;
+    J.cv = in->CV;
+    J.x = x;
+    J.psi0 = psi0;
+    J.work = work.data();
+    tdse_ground_state_host(J);
+    (void)gs_status(J, "");  // a limit cycle or a zero pivot is reported; the last iterate is kept
+    in->x = x;
+    in->psi0 = psi0;
+    in->Einit = J.energy;
 }
 
 void call1DTDSE(inputs_def *inputs, outputs_def *outputs)
@@ -1282,18 +1465,13 @@ void call1DTDSE(inputs_def *inputs, outputs_def *outputs)
         n_snap = nt_out / spd;
     }
 
-    ctdse_geometry *G = nullptr;
-    int st = geometry_from(in.num_r, in.dx, dt, in.trg.a, in.Eguess, in.CV, in.x_int, in.psi0, in.x, &G);
-    if (st != CTDSE_OK) {
-        delete G;
-        return;
-    }
+    int st = CTDSE_OK;
+    const std::shared_ptr<ctdse_geometry> geom = cached_geometry(in, dt, &st);
+    if (!geom) return;
+    ctdse_geometry *G = geom.get();
     const size_t nx = in.num_r + 1;
     DevMem fieldbuf, outbuf, snaps, tgrid;
-    auto fail_dev = [&](hipError_t e) {
-        set_status(CTDSE_ERR_DEVICE, std::string("call1DTDSE: ") + hipGetErrorString(e));
-        delete G;
-    };
+    auto fail_dev = [&](hipError_t e) { set_status(CTDSE_ERR_DEVICE, std::string("call1DTDSE: ") + hipGetErrorString(e)); };
     hipError_t e = fieldbuf.alloc((size_t)(n2 + 3) * sizeof(double));
     if (e == hipSuccess) e = hipMemset(fieldbuf.p, 0, (size_t)(n2 + 3) * sizeof(double));
     if (e != hipSuccess) return fail_dev(e);
@@ -1337,10 +1515,7 @@ void call1DTDSE(inputs_def *inputs, outputs_def *outputs)
     const ctdse_launch_config single{TDSE_MAX_WAVES, 1, 1};
     st = std::getenv("MMA_TDSE_NO_STEP_PARALLEL") ? (int)CTDSE_ERR_UNSUPPORTED : propagate_steps(s, 0);
     if (st == CTDSE_ERR_UNSUPPORTED) st = propagate({s}, 0, &single);
-    if (st != CTDSE_OK) {
-        delete G;
-        return;
-    }
+    if (st != CTDSE_OK) return;
     hipLaunchKernelGGL(tdse_tgrid_kernel, dim3(1), dim3(64), 0, 0, tgrid.as<double>(), nt_out, in.tmin, dt);
 
     // outputs: library-owned host arrays sized as singleTDSE.c:95-100 (Nt + 2 with Nt = n2 + 1)
@@ -1360,12 +1535,18 @@ void call1DTDSE(inputs_def *inputs, outputs_def *outputs)
         st = spectrum_to_host(o, nt_out, dt, tmax, &outputs->omegagrid, &outputs->Fsourceterm,
                               &outputs->FsourcetermM2);
     outputs->Nomega = nt_out / 2 + 1;
+    bool copied = outputs->tgrid && outputs->sourceterm && outputs->PopTot && outputs->PopInt && outputs->expval &&
+                  outputs->Efield;
     if (in.analy.writewft) {
-        outputs->psi = (double **)malloc(sizeof(double *) * std::max(n_snap, 1));
-        for (int r = 0; r < n_snap; r++)
+        outputs->psi = (double **)calloc(std::max(n_snap, 1), sizeof(double *));
+        copied = copied && outputs->psi;
+        for (int r = 0; outputs->psi && r < n_snap; r++) {
             outputs->psi[r] = host_copy(snaps.as<double>() + (size_t)r * 2 * nx, 2 * nx, 2 * nx);
+            copied = copied && outputs->psi[r];
+        }
     }
-    delete G;
+    // a failed copy leaves a NULL output: report it (the spectra set their own status)
+    if (st == CTDSE_OK && !copied) set_status(CTDSE_ERR_DEVICE, "call1DTDSE: allocating or copying the outputs failed");
 }
 
 void set_time_and_field(inputs_def *in, double *time, double *field, int N)

@@ -138,6 +138,10 @@ __global__ void tdse_ground_state_kernel(GsJob *jobs, int n_jobs);
 __global__ void tdse_tables_grid_kernel(TableJob *job);
 __global__ void tdse_tables_pivot_kernel(TableJob *job);
 __global__ void tdse_tables_chunk_kernel(TableJob *job);
+__global__ void tdse_tables_ib_kernel(TableJob *jobs);
+// host forms of the serial setup steps (tdse_setup.hip), bit-identical to the kernels
+void tdse_ground_state_host(GsJob &J);
+void tdse_tables_pivot_host(TableJob &J);
 __global__ void tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
                                       int *counter, double2 *work, long long work_stride);
 __global__ void tdse_propagate_pair_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
@@ -154,7 +158,6 @@ __global__ void tdse_adjoint_kernel(const TdseGeom *geoms, const TdseAdjInst *in
 __global__ void tdse_reverse_adjoint_kernel(const TdseGeom *geoms, const TdseRevInst *insts, const int *order,
                                             int n_inst, int *counter, double2 *work, long long work_stride);
 __global__ void tdse_fill_int_kernel(int *p, int n, int v);
-__global__ void tdse_stream_copy_kernel(const double2 *src, double2 *dst, long long n);
 __global__ void tdse_xgrid_kernel(double *x, int num_r, double dx);
 __global__ void tdse_tgrid_kernel(double *tgrid, int n, double tmin, double dt);
 __global__ void tdse_dft_r2c_kernel(const double *sig, int n, int nbins, double2 *out);

@@ -9,10 +9,10 @@
 
 namespace {
 
-__device__ inline double soft_coulomb(double x, double a) { return -1.0 / __builtin_sqrt(a * a + x * x); }
+__host__ __device__ inline double soft_coulomb(double x, double a) { return -1.0 / __builtin_sqrt(a * a + x * x); }
 
 // tridiag.c:26-70, one lane: complex Thomas with the reference's operation order.
-__device__ int thomas(const double *sub, const double *dia, const double *sup, const double *rhs, double *u, int n,
+__host__ __device__ int thomas(const double *sub, const double *dia, const double *sup, const double *rhs, double *u, int n,
                       double *gam)
 {
     int status = 0;
@@ -51,14 +51,14 @@ __device__ int thomas(const double *sub, const double *dia, const double *sup, c
     return status;
 }
 
-__device__ double sumsq(const double *v, int num_r)
+__host__ __device__ double sumsq(const double *v, int num_r)
 {
     double s = 0.;
     for (int i = 0; i <= num_r; i++) s = s + v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1];
     return s;
 }
 
-__device__ void normalise(double *v, int num_r)
+__host__ __device__ void normalise(double *v, int num_r)
 {
     const double nrm = __builtin_sqrt(sumsq(v, num_r));
     for (int i = 0; i <= num_r; i++) {
@@ -68,7 +68,7 @@ __device__ void normalise(double *v, int num_r)
 }
 
 // <M2 psi, H psi> / <M2 psi, M2 psi>, tridiag.c:172-238; V holds soft_coulomb(x_j).
-__device__ double energy(const double *psi, double c, const double *V, int n, double *h, double *m)
+__host__ __device__ double energy(const double *psi, double c, const double *V, int n, double *h, double *m)
 {
     h[0] = c * psi[0] - 0.5 * c * psi[2];
     h[0] = h[0] + ((10 / 12.) * psi[0] * (V[0]) + (1 / 12.) * psi[2] * (V[1]));
@@ -102,16 +102,10 @@ __device__ double energy(const double *psi, double c, const double *V, int n, do
     return e / nrm;
 }
 
-} // namespace
-
-// Inverse iteration for the ground state, tools.c:31-51 and tridiag.c:86-159: one job per
-// workgroup, run by its first lane (the iteration is a serial chain; many geometries run
-// side by side, one per wave).
-__global__ void tdse_ground_state_kernel(GsJob *jobs, int n_jobs)
+// Inverse iteration for the ground state of one job, tools.c:31-51 and tridiag.c:86-159: a
+// serial chain, run by one GPU lane per job or on the host for one geometry alone.
+__host__ __device__ void gs_iterate(GsJob &J)
 {
-    const int id = blockIdx.x;
-    if (id >= n_jobs || threadIdx.x != 0) return;
-    GsJob &J = jobs[id];
     const int n = J.num_r;
     const long long sz = 2 * (long long)(n + 1);
     double *V = J.work;                 // n + 2
@@ -173,6 +167,22 @@ __global__ void tdse_ground_state_kernel(GsJob *jobs, int n_jobs)
     J.status = status;
 }
 
+} // namespace
+
+// One ground-state job per workgroup, run by its first lane (many geometries side by side,
+// one per wave).
+__global__ void tdse_ground_state_kernel(GsJob *jobs, int n_jobs)
+{
+    const int id = blockIdx.x;
+    if (id >= n_jobs || threadIdx.x != 0) return;
+    gs_iterate(jobs[id]);
+}
+
+// The same on the host (arrays of J in host memory, work of 16 (num_r + 2) doubles): the same
+// operations in the same order without contraction, so the same bits as the GPU lane and
+// the reference's CPU code.
+void tdse_ground_state_host(GsJob &J) { gs_iterate(J); }
+
 // Tables, part 1 (blocks along x, one job per blockIdx.y): g_j, V'(x_j) and psi0 in tiled order.
 __global__ void tdse_tables_grid_kernel(TableJob *jobs)
 {
@@ -207,14 +217,13 @@ __global__ void tdse_tables_grid_kernel(TableJob *jobs)
     }
 }
 
-// Tables, part 2 (one lane): pivots 1/beta_j of B' (tridiag.c:38-47 on the scaled
-// matrix), truncation decay rates, PopInt window (prop.c:86-93 with
-// tools_algorithmic.c:19-43) and the psi0 support window.
-// One job per workgroup (blockIdx.x), first lane.
-__global__ void tdse_tables_pivot_kernel(TableJob *jobs)
+namespace {
+
+// Tables, part 2 (serial): pivots 1/beta_j of B' (tridiag.c:38-47 on the scaled matrix) in
+// natural order into J.pivot_work, truncation decay rates, PopInt window (prop.c:86-93 with
+// tools_algorithmic.c:19-43) and the psi0 support window.  J.x and J.psi0_nat are read.
+__host__ __device__ void pivot_scan(TableJob &J)
 {
-    if (threadIdx.x != 0) return;
-    TableJob &J = jobs[blockIdx.x];
     const int nx = J.num_r + 1;
     const double k36 = 36.0 * J.dt / (J.dx * J.dx);
     const double gk = (-3.0 * J.dt) / (J.dx * J.dx);
@@ -249,7 +258,6 @@ __global__ void tdse_tables_pivot_kernel(TableJob *jobs)
             rho_b = fmax(rho_b, __builtin_sqrt(1.0 + gp * gp) * inv_abs);
             rho_b = fmax(rho_b, __builtin_sqrt(1.0 + g0 * g0) * inv_abs);
         }
-        J.ib[tdse_index(j)] = make_double2(ir, ii);
     }
     J.rho_fwd = rho_f;
     J.rho_bwd = rho_b;
@@ -281,6 +289,27 @@ __global__ void tdse_tables_pivot_kernel(TableJob *jobs)
     }
     J.p0_lo = lo;
     J.p0_hi = hi;
+}
+
+} // namespace
+
+// Part 2 on the GPU: one job per workgroup (blockIdx.x), first lane.
+__global__ void tdse_tables_pivot_kernel(TableJob *jobs)
+{
+    if (threadIdx.x != 0) return;
+    pivot_scan(jobs[blockIdx.x]);
+}
+
+// Part 2 on the host (J.x, J.psi0_nat and J.pivot_work in host memory), the same bits.
+void tdse_tables_pivot_host(TableJob &J) { pivot_scan(J); }
+
+// Tables, part 2b (one thread per point, one job per blockIdx.y): the pivots in tiled order.
+__global__ void tdse_tables_ib_kernel(TableJob *jobs)
+{
+    TableJob &J = jobs[blockIdx.y];
+    const int nx = J.num_r + 1;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nx; j += gridDim.x * blockDim.x)
+        J.ib[tdse_index(j)] = make_double2(J.pivot_work[2 * j], J.pivot_work[2 * j + 1]);
 }
 
 // Tables, part 3 (one thread per chunk, one job per blockIdx.y): transfer products of the
@@ -324,22 +353,6 @@ __global__ void tdse_xgrid_kernel(double *x, int num_r, double dx)
     if (k > num_r) return;
     const double half_span = 0.5 * num_r * dx;
     x[k] = (double)k * dx - half_span;
-}
-
-// Streaming copy dst[i] = src[i] in 16-byte lanes, four in flight per lane, grid-stride: the
-// read-and-write bandwidth probe the propagation kernel's traffic is compared with.
-__global__ void tdse_stream_copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n)
-{
-    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
-        const double2 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-        dst[i] = a;
-        dst[i + stride] = b;
-        dst[i + 2 * stride] = c;
-        dst[i + 3 * stride] = d;
-    }
-    for (; i < n; i += stride) dst[i] = src[i];
 }
 
 // A tiled state in natural order, interleaved re, im (a snapshot row of prop.c:248-263)

@@ -110,4 +110,26 @@ def outputs(res):
     return out
 
 
-__all__ = ["screen", "outputs", "find_interval", "omega_au_of_wavelength", "OMEGA_AU2SI", "OUTPUTS"]
+def save(path, res, keep=None):
+    """Writes the datasets of `outputs(res)` to the HDF5 file `path` as the driver writes
+    results_Hankel.h5 (:307-365): each with its units attribute ('[arb. u.]' for the fields,
+    '[SI]' for the grids, mynumerics.adddataset).  keep: an HDF5 file whose datasets and
+    attributes are carried over (datasets of the same name are replaced)."""
+    from . import h5lite
+    data, attrs = ({}, {}) if keep is None else h5lite.read_all(keep)
+    for key, value in outputs(res).items():
+        p = "/" + key
+        data[p] = np.asarray(value)
+        attrs[p] = {"units": b"[SI]" if key.rsplit("/", 1)[-1] in ("ogrid", "rgrid", "zgrid") else b"[arb. u.]"}
+    h5lite.write(path, data, attrs)
+
+
+def load(path):
+    """The Hankel/outputs datasets of an HDF5 file as {path: array} (complex fields as (..., 2)
+    re/im, as stored)."""
+    from . import h5lite
+    f = h5lite.H5File(path)
+    return {OUTPUTS + "/" + k: f[OUTPUTS + "/" + k] for k in f.keys(OUTPUTS)}
+
+
+__all__ = ["screen", "outputs", "save", "load", "find_interval", "omega_au_of_wavelength", "OMEGA_AU2SI", "OUTPUTS"]

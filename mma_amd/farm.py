@@ -95,17 +95,17 @@ def spectra(signal, dt, tmax):
 
 
 def run(output_field, tgrid, rgrid, zgrid, kz_step, nz_max, kr_step, nr_max, num_r, dx, dt, a,
-        e_guess=-0.5792, cv=1e-25, x_int=2.0, tmin=0.0, config=None, device="cuda"):
+        e_guess=-0.5792, cv=1e-25, x_int=2.0, tmin=0.0, config=None, device="cuda", interp_by_nt=0, ntinterp=0):
     """All (z, r) instances of a CUPRAD field through the TDSE in one launch.
 
     output_field: (z, t, r) field in SI units (V/m), tgrid in s, rgrid/zgrid in m (the CUPRAD
     outputs); kz_step/nz_max/kr_step/nr_max: the coarsening of CTDSE/inputs; the rest as
-    inputs_def (num_r, dx, dt, trg.a, Eguess, CV, x_int, tmin).  config: launch shape (default:
-    the library's).  Returns a dict keyed like the merged file (OUTPUTS + '/SourceTerm', ...),
-    torch tensors on the device for the per-instance data."""
+    inputs_def (num_r, dx, dt, trg.a, Eguess, CV, x_int, tmin, InterpByDTorNT, Ntinterp).
+    config: launch shape (default: the library's).  Returns a dict keyed like the merged file
+    (OUTPUTS + '/SourceTerm', ...), torch tensors on the device for the per-instance data."""
     fields, nz_c, nr_c = coarse_fields(output_field, kz_step, nz_max, kr_step, nr_max)
     t_au = np.asarray(tgrid, dtype=np.float64) / TIMEAU  # MPI-TDSE.c:128-130
-    k1, dt_ref, field_dt = _refine(t_au, dt)
+    k1, dt_ref, field_dt = _refine(t_au, dt, interp_by_nt, ntinterp)
     tmax = t_au[-1] - t_au[0]  # singleTDSE.c:58
     n_e = fields.shape[1]
     n2 = k1 * n_e
@@ -151,9 +151,15 @@ def run(output_field, tgrid, rgrid, zgrid, kz_step, nz_max, kr_step, nr_max, num
     return res
 
 
-def _refine(t_au, dt):
-    """k1 rule of singleTDSE.c:52-68 (InterpByDTorNT = 0): refined step and factor."""
+def _refine(t_au, dt, interp_by_nt=0, ntinterp=0):
+    """k1 rule of singleTDSE.c:52-68: k1 = Ntinterp + 1 when InterpByDTorNT is 1, else from the
+    field's step and dt; returns (k1, refined step, field step)."""
     field_dt = t_au[1] - t_au[0]
+    if interp_by_nt == 1:
+        k1 = int(ntinterp) + 1
+        if k1 < 1:
+            raise ValueError("field refinement factor k1 < 1 (Ntinterp < 0)")
+        return k1, field_dt / float(k1), field_dt
     k1 = int(math.floor(field_dt / dt))
     if k1 != 1:
         k1 += 1
@@ -168,5 +174,94 @@ def as_host(result):
     return {k: (v.cpu().numpy() if isinstance(v, torch.Tensor) else v) for k, v in result.items()}
 
 
-__all__ = ["run", "coarse_fields", "coarsen_grid", "instance_index", "four_interp", "spectra", "as_host",
-           "TIMEAU", "EFIELDAU", "OUTPUTS"]
+# -- archives (HDF5, through mma_amd.h5lite) ---------------------------------------------------
+
+CUPRAD_OUTPUTS = "CUPRAD/outputs"  # 1DTDSE/sources/h5namelist.h:13-21
+CTDSE_INPUTS = "CTDSE/inputs"
+GLOBAL_INPUTS = "global_inputs"
+# units attached to the merged outputs (1DTDSE/python/merge.py:38-56); PopTot, which the merge
+# leaves out, is written too
+UNITS = {"Efield": "[a.u.]", "FEfield": "[a.u.]", "SourceTerm": "[a.u.]", "FSourceTerm": "[a.u.]",
+         "FEfieldM2": "[a.u.]", "FSourceTermM2": "[a.u.]", "PopInt": "[-]", "PopTot": "[-]", "expval_x": "[a.u.]",
+         "tgrid": "[a.u.]", "omegagrid": "[a.u.]", "Energy_of_the_ground_state": "[a.u.]", "xgrid_micro": "[a.u.]",
+         "ground_state": "[a.u.]", "zgrid_coarse": "[a.u.]", "rgrid_coarse": "[a.u.]", "trg_a": "[a.u.]"}
+# soft-Coulomb presets of the gas_preset input (1DTDSE/sources/constants.c:49-116)
+GAS_PRESETS = {"He": (0.6950, -0.9036), "Ne": (0.8161, -0.7924), "Ar": (1.1893, -0.5792), "Kr": (1.3676, -0.5145),
+               "Xe": (1.6171, -0.4458)}
+
+
+def _scalar(f, path):
+    return np.asarray(f[path]).reshape(-1)[0].item()
+
+
+def read_inputs(f):
+    """The CTDSE inputs of an archive (an h5lite.H5File) as ReadInputs reads them
+    (1DTDSE/sources/tools_hdf5.c:65-148), with the coarsening of MPI-TDSE.c:112-116: num_r =
+    2 Nx_max, dx, InterpByDTorNT, dt, Ntinterp, x_int, the soft-Coulomb a and E guess of
+    global_inputs/gas_preset unless CTDSE/inputs/trg_a and Eguess are given (those take
+    precedence), CV_criterion_of_GS, kz_step, kr_step, Nz_max, Nr_max.  ValueError where the
+    reference stops: neither a preset nor trg_a / Eguess, or an unknown preset."""
+    P = CTDSE_INPUTS + "/"
+    d = {"num_r": 2 * int(_scalar(f, P + "Nx_max")), "dx": float(_scalar(f, P + "dx")),
+         "interp_by_nt": int(_scalar(f, P + "InterpByDTorNT")), "dt": float(_scalar(f, P + "dt")),
+         "ntinterp": int(_scalar(f, P + "Ntinterp")), "x_int": float(_scalar(f, P + "x_int")),
+         "cv": float(_scalar(f, P + "CV_criterion_of_GS"))}
+    a = e = None
+    preset = GLOBAL_INPUTS + "/gas_preset"
+    if preset in f:
+        name = _scalar(f, preset)
+        name = (name.decode() if isinstance(name, bytes) else str(name)).rstrip("\x00").strip()
+        if name not in GAS_PRESETS:
+            raise ValueError("Soft-coulomb parameter for this input not supported: %r" % name)
+        a, e = GAS_PRESETS[name]
+    if P + "trg_a" in f:
+        a = float(_scalar(f, P + "trg_a"))
+    if P + "Eguess" in f:
+        e = float(_scalar(f, P + "Eguess"))
+    if a is None:
+        raise ValueError("The soft-Coulomb parameter in 1D-TDSE not specified.")
+    if e is None:
+        raise ValueError("The ground-state-energy guess in 1D-TDSE not specified.")
+    d["a"], d["e_guess"] = a, e
+    for k in ("kz_step", "kr_step", "Nz_max", "Nr_max"):
+        d[k] = int(_scalar(f, P + k))
+    return d
+
+
+def run_archive(archive, out=None, config=None, device="cuda"):
+    """The whole task farm on an MMA archive (MPI-TDSE.c:44-329 and merge.py): reads
+    CUPRAD/outputs/{output_field (z, t, r), tgrid, rgrid, zgrid} and the CTDSE inputs, runs every
+    coarsened instance in one launch, and writes CTDSE/outputs/* in the merged layout with units
+    attributes to `out`; out None: into the archive itself (rewritten with everything it held;
+    outputs of an earlier run are replaced).  Returns the result mapping of `run` (device
+    tensors)."""
+    from . import h5lite
+    f = h5lite.H5File(archive)
+    inp = read_inputs(f)
+    C = CUPRAD_OUTPUTS + "/"
+    res = run(f[C + "output_field"], f[C + "tgrid"], f[C + "rgrid"], f[C + "zgrid"], inp["kz_step"], inp["Nz_max"],
+              inp["kr_step"], inp["Nr_max"], inp["num_r"], inp["dx"], inp["dt"], inp["a"], e_guess=inp["e_guess"],
+              cv=inp["cv"], x_int=inp["x_int"], config=config, device=device, interp_by_nt=inp["interp_by_nt"],
+              ntinterp=inp["ntinterp"])
+    del f
+    write_outputs(archive if out is None else out, res, keep=archive if out in (None, archive) else None)
+    return res
+
+
+def write_outputs(path, result, keep=None):
+    """Writes the merged outputs (`run`'s mapping; device tensors are copied to the host) to the
+    HDF5 file `path` with their units attributes.  keep: an HDF5 file whose datasets and
+    attributes are carried over (the archive the outputs are added to); datasets of the same
+    name are replaced."""
+    from . import h5lite
+    data, attrs = ({}, {}) if keep is None else h5lite.read_all(keep)
+    for key, value in as_host(result).items():
+        p = "/" + key
+        data[p] = np.asarray(value)
+        attrs[p] = {"units": UNITS.get(key.rsplit("/", 1)[-1], "[a.u.]").encode()}
+    h5lite.write(path, data, attrs)
+
+
+__all__ = ["run", "run_archive", "read_inputs", "write_outputs", "coarse_fields", "coarsen_grid", "instance_index",
+           "four_interp", "spectra", "as_host", "TIMEAU", "EFIELDAU", "OUTPUTS", "UNITS", "GAS_PRESETS"]
+

@@ -129,3 +129,50 @@ def test_farm_output_feeds_the_hankel_provider():
     assert len(planes) == 5  # the provider drops the last plane (Hankel_transform.py:62)
     for k, pl in enumerate(planes):
         np.testing.assert_array_equal(pl, (FS[k, :, 3:40, 0] + 1j * FS[k, :, 3:40, 1]).T)
+
+
+@pytest.mark.gpu
+def test_archive_run_writes_the_merged_layout(tmp_path):
+    """The farm on an MMA archive: CUPRAD/outputs (float32 (z, t, r) field and its grids), CTDSE
+    inputs with a gas preset and Ntinterp refinement (InterpByDTorNT = 1, Ntinterp = 1: k1 = 2),
+    run_archive writing CTDSE/outputs/* into the archive itself; the written outputs equal
+    farm.run's on the same inputs bit for bit, carry their units, keep the archive's other
+    datasets, and feed the Hankel provider as the reference's merged file does; the far field
+    of those sources is written to a results file and read back."""
+    from mma_amd import far_field as FF, h5lite
+    field, tgrid, r, z = _cuprad_like(nz=4, nr=3, dt_field_au=0.5)
+    p = str(tmp_path / "results.h5")
+    d = {"/CUPRAD/outputs/output_field": field, "/CUPRAD/outputs/tgrid": tgrid, "/CUPRAD/outputs/rgrid": r,
+         "/CUPRAD/outputs/zgrid": z, "/global_inputs/gas_preset": b"Ar",
+         "/CTDSE/inputs/Nx_max": np.int32(800), "/CTDSE/inputs/dx": 0.1, "/CTDSE/inputs/InterpByDTorNT": np.int32(1),
+         "/CTDSE/inputs/dt": 0.25, "/CTDSE/inputs/Ntinterp": np.int32(1), "/CTDSE/inputs/x_int": 2.0,
+         "/CTDSE/inputs/CV_criterion_of_GS": 1e-25, "/CTDSE/inputs/kz_step": np.int32(1),
+         "/CTDSE/inputs/kr_step": np.int32(1), "/CTDSE/inputs/Nz_max": np.int32(4), "/CTDSE/inputs/Nr_max": np.int32(3)}
+    h5lite.write(p, d, {"/CUPRAD/outputs/output_field": {"units": b"[V/m]"}})
+    res = farm.as_host(farm.run_archive(p))
+    ref = farm.as_host(farm.run(field, tgrid, r, z, 1, 4, 1, 3, 1600, 0.1, 0.25, W.GRID["a"], e_guess=W.GRID["e_guess"],
+                                cv=1e-25, x_int=2.0, interp_by_nt=1, ntinterp=1))
+    f = h5lite.H5File(p)
+    P = farm.OUTPUTS
+    for key in ref:
+        np.testing.assert_array_equal(res[key], ref[key])
+        np.testing.assert_array_equal(f["/" + key], ref[key])
+    assert f.attrs("/" + P + "/PopInt")["units"] == b"[-]" and f.attrs("/" + P + "/SourceTerm")["units"] == b"[a.u.]"
+    np.testing.assert_array_equal(f["/CUPRAD/outputs/output_field"], field)
+    assert f.attrs("/CUPRAD/outputs/output_field")["units"] == b"[V/m]"
+    n_e = len(tgrid)
+    assert f["/" + P + "/SourceTerm"].shape == (4, 3, 2 * n_e + 2)  # k1 = 2: outputs.Nt = k1 N_E + 2
+    # the far field of the written sources (vacuum), read through the file as the driver does
+    w0 = FF.omega_au_of_wavelength(800e-9)
+    out = FF.screen(f["/" + P + "/omegagrid"], f["/" + P + "/rgrid_coarse"], f["/" + P + "/zgrid_coarse"], f, w0,
+                    [3.0, 9.0], rmax_FF=4e-3, Nr_FF=16, distance_FF=1.0, include_absorption=False,
+                    include_dispersion=False)
+    mem = FF.screen(ref[P + "/omegagrid"], ref[P + "/rgrid_coarse"], ref[P + "/zgrid_coarse"], ref, w0, [3.0, 9.0],
+                    rmax_FF=4e-3, Nr_FF=16, distance_FF=1.0, include_absorption=False, include_dispersion=False)
+    np.testing.assert_array_equal(out.FF_integrated, mem.FF_integrated)
+    q = str(tmp_path / "results_Hankel.h5")
+    FF.save(q, out)
+    back = FF.load(q)
+    np.testing.assert_array_equal(back[FF.OUTPUTS + "/FF_integrated"][..., 0], out.FF_integrated.real)
+    np.testing.assert_array_equal(back[FF.OUTPUTS + "/ogrid"], out.ogrid)
+    assert h5lite.H5File(q).attrs("/" + FF.OUTPUTS + "/rgrid")["units"] == b"[SI]"

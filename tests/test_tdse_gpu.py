@@ -16,11 +16,11 @@ TOL = 1e-8
 SERIES = ["sourceterm", "PopTot", "PopInt", "expval"]
 
 
-def run_case(dll, g, writewft=0, tprint=10.0):
+def run_case(dll, g, writewft=0, tprint=10.0, x_int=None):
     inputs = inputs_def()
     inputs.init_default_inputs(Eguess=float(g["e_guess"]), num_r=int(g["num_r"]), dx=float(g["dx"]),
-                               dt=float(g["dt"]), trg_a=float(g["a"]), CV=float(g["cv"]), x_int=float(g["x_int"]),
-                               writewft=writewft, tprint=tprint)
+                               dt=float(g["dt"]), trg_a=float(g["a"]), CV=float(g["cv"]),
+                               x_int=float(g["x_int"]) if x_int is None else x_int, writewft=writewft, tprint=tprint)
     inputs.init_time_and_field(dll, t=g["t"], E=g["field"])
     dll.init_GS(inputs)
     outputs = outputs_def()
@@ -75,6 +75,42 @@ def test_single_instance_800nm(dll):
     check_series(outputs, g)
     outputs.delete(dll)
     inputs.delete(dll)
+
+
+def test_call1dtdse_reuses_its_geometry_only_for_equal_inputs(dll):
+    """call1DTDSE keeps the geometry of its last call: a repeated call on the same inputs gives
+    the same bits; a call with another PopInt window (x_int) or with a ground state changed in
+    one bit builds a new geometry, checked on the rows that change."""
+    g = golden("ref_single.npz")
+    inputs, first = run_case(dll, g)
+    again = outputs_def()
+    dll.call1DTDSE(inputs, again)
+    for k in SERIES + ["Efield"]:
+        np.testing.assert_array_equal(getattr(again, "get_" + k)(), getattr(first, "get_" + k)())
+    # another PopInt window: same dipole, other PopInt
+    inputs2, wide = run_case(dll, g, x_int=5.0)
+    np.testing.assert_array_equal(wide.get_sourceterm(), first.get_sourceterm())
+    assert np.max(np.abs(wide.get_PopInt() - first.get_PopInt())) > 1e-6
+    assert wide.get_PopInt()[0] > first.get_PopInt()[0]
+    # back to the first inputs: the first results again
+    third = outputs_def()
+    dll.call1DTDSE(inputs, third)
+    np.testing.assert_array_equal(third.get_PopInt(), first.get_PopInt())
+    # a ground state perturbed in its last bit changes the geometry (psi0 is compared by value)
+    psi = inputs.get_GS()
+    from ctypes import c_double
+    import ctypes
+    buf = ctypes.cast(inputs.psi0, ctypes.POINTER(c_double))
+    j = int(np.argmax(np.abs(psi)))
+    buf[2 * j] = np.nextafter(buf[2 * j], 2.0)
+    fourth = outputs_def()
+    dll.call1DTDSE(inputs, fourth)
+    assert not np.array_equal(fourth.get_PopTot(), first.get_PopTot())
+    assert rel_err(fourth.get_sourceterm(), first.get_sourceterm()) < 1e-8
+    for o in (first, again, wide, third, fourth):
+        o.delete(dll)
+    inputs.delete(dll)
+    inputs2.delete(dll)
 
 
 def test_single_instance_paths_agree(dll, monkeypatch):
