@@ -209,6 +209,25 @@ int ctdse_propagate_batch(int n_inst, const ctdse_geometry *const *geom, const i
 int ctdse_propagate_one(const ctdse_geometry *geom, int n_steps, const double *field, double *out, void *stream);
 
 /*
+ * ctdse_propagate_one without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results (bit for bit) as
+ * ctdse_propagate_one; its scratch (two state vectors, the gated field, per-segment partial
+ * sums, the descriptors) lives in a fixed device array of CTDSE_NOALLOC_SCRATCH_BYTES that
+ * each GPU the library runs on holds until the process ends, and it calls no hipMalloc,
+ * hipFree or allocating library routine.  A run needs about
+ * 32 * ctdse_state_length(geom) + 8 * (n_steps + 1) bytes plus a few KiB: larger runs fail
+ * with CTDSE_ERR_ARGUMENT; a geometry whose truncation halo leaves no room for the
+ * step-parallel solve fails with CTDSE_ERR_UNSUPPORTED.  It queues the work on `stream` and
+ * returns without waiting.  Every call on one GPU shares that GPU's array and leaves its
+ * contents behind (each call initialises what it reads): calls that overlap on one GPU, on
+ * other streams or from other threads, race; order them on one stream.
+ */
+#define CTDSE_NOALLOC_SCRATCH_BYTES (4 << 20)
+int ctdse_propagate_one_noalloc(const ctdse_geometry *geom, int n_steps, const double *field, double *out,
+                                void *stream);
+
+/*
  * Many geometries at once (no counterpart in the reference; the task farm of
  * MPI-TDSE.c:145 computes one ground state for all its instances): geometry i has
  * (num_r[i], dx[i], dt[i], a[i], e_guess[i], cv[i], x_int[i]) and its ground state by

@@ -15,6 +15,11 @@
 #include <string>
 #include <vector>
 
+// Device memory of ctdse_propagate_one_noalloc, one copy per GPU for the life of the process.
+// Host code never uses this name as an address: the device address comes from
+// hipGetSymbolAddress (scratch_address below).
+static __device__ __align__(256) unsigned char scratch[CTDSE_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local int g_status = CTDSE_OK;
@@ -911,6 +916,31 @@ int propagate_steps(const InstSpec &s, hipStream_t stream)
     return queued;
 }
 
+// The device address of this GPU's `scratch`, looked up once per device with that device
+// current (each GPU holds its own copy of the array at its own address).
+std::mutex g_scratch_mutex;
+constexpr int kMaxDevices = 64;
+void *g_scratch_addr[kMaxDevices] = {};
+
+int scratch_address(void **out)
+{
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_status(CTDSE_ERR_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_status(CTDSE_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return CTDSE_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
 // Plan path: everything allocated and uploaded here, once.
 int make_plan(const std::vector<InstSpec> &spec, const ctdse_launch_config *cfg, ctdse_batch_plan &P)
 {
@@ -1146,6 +1176,51 @@ int ctdse_propagate_one(const ctdse_geometry *geom, int n_steps, const double *f
     if (st != CTDSE_ERR_UNSUPPORTED) return st;
     const ctdse_launch_config single{TDSE_MAX_WAVES, 1, 1};
     return propagate({s}, (hipStream_t)stream, &single);
+}
+
+int ctdse_propagate_one_noalloc(const ctdse_geometry *geom, int n_steps, const double *field, double *out,
+                                void *stream)
+{
+    clear_status();
+    if (!geom || !field || !out || n_steps < 1) return set_status(CTDSE_ERR_ARGUMENT, "null or empty argument");
+    if (device_ready() != CTDSE_OK) return g_status;
+    InstSpec s;
+    s.geom = geom;
+    s.n_steps = n_steps;
+    s.field = field;
+    s.out = out;
+    int nw = 0, nseg = 0;
+    if (steps_shape(s, &nw, &nseg) != CTDSE_OK)
+        return set_status(CTDSE_ERR_UNSUPPORTED, "the geometry's truncation halo leaves no room for the step-parallel "
+                                                 "solve, which is the only form without a work buffer");
+    const TdseGeom &g = geom->dev;
+    // layout of the scratch array: descriptors, partial sums, gated field, two states
+    size_t off = 0;
+    const size_t o_geom = off;
+    off += align256(sizeof(TdseGeom));
+    const size_t o_inst = off;
+    off += align256(sizeof(TdseInst));
+    const size_t o_part = off;
+    off += align256(2 * (size_t)nseg * 5 * sizeof(double));
+    const size_t o_gated = off;
+    off += align256((size_t)(n_steps + 1) * sizeof(double));
+    const size_t o_psi = off;
+    off += 2 * (size_t)g.npad * sizeof(double2);
+    if (off > (size_t)CTDSE_NOALLOC_SCRATCH_BYTES)
+        return set_status(CTDSE_ERR_ARGUMENT, "ctdse_propagate_one_noalloc: this run needs " + std::to_string(off) +
+                                                  " bytes of scratch (two states of " + std::to_string(g.npad) +
+                                                  " complex doubles, " + std::to_string(n_steps + 1) +
+                                                  " field samples), more than the " +
+                                                  std::to_string((size_t)CTDSE_NOALLOC_SCRATCH_BYTES) + " it holds");
+    void *base = nullptr;
+    if (scratch_address(&base) != CTDSE_OK) return g_status;
+    char *p = static_cast<char *>(base);
+    double2 *psi = reinterpret_cast<double2 *>(p + o_psi);
+    const StepBufs b{reinterpret_cast<TdseGeom *>(p + o_geom), reinterpret_cast<TdseInst *>(p + o_inst),
+                     reinterpret_cast<double *>(p + o_part), reinterpret_cast<double *>(p + o_gated),
+                     {psi, psi + g.npad}};
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    return enqueue_steps(s, nw, nseg, b, (hipStream_t)stream);
 }
 
 int ctdse_geometry_create_batch(int n, const int *num_r, const double *dx, const double *dt, const double *a,

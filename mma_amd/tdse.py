@@ -53,6 +53,8 @@ def _lib():
                                             c_void_p, POINTER(c_longlong), c_void_p]
         L.ctdse_propagate_one.restype = c_int
         L.ctdse_propagate_one.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p]
+        L.ctdse_propagate_one_noalloc.restype = c_int
+        L.ctdse_propagate_one_noalloc.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p]
         L.ctdse_geometry_create_batch.restype = c_int
         L.ctdse_geometry_create_batch.argtypes = [c_int, POINTER(c_int)] + [POINTER(c_double)] * 6 + [
             POINTER(c_void_p), POINTER(c_int)]
