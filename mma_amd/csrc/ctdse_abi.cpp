@@ -10,8 +10,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -84,6 +86,14 @@ constexpr double kTruncTol = 1e-17;   // relative truncation of the segmented so
 
 } // namespace
 
+// Lattice tables shared by geometries of one batch (TdseGeom::sg): g, V' and 1/beta of the
+// widest geometry in eight alignments (shift 0..7 points), so that every geometry's chunks
+// start on a shared chunk boundary.
+struct Lattice {
+    static constexpr int kAlign = TDSE_CHUNK;
+    DevMem g[kAlign], dpot[kAlign], ib[kAlign], chq[kAlign], chg[kAlign];
+};
+
 struct ctdse_geometry {
     TdseGeom dev{};
     int num_r = 0;
@@ -91,6 +101,7 @@ struct ctdse_geometry {
     double e_init = 0;
     int gs_iters = 0;
     DevMem x, psi0_nat, g, dpot, ib, psi0, chq, chg, adq, adg;
+    std::shared_ptr<Lattice> lattice;  // null, or the batch's shared tables this geometry reads
 };
 
 namespace {
@@ -379,6 +390,59 @@ int upload_grid(ctdse_geometry *G, const HostGrid &hg, const std::string &who)
     return CTDSE_OK;
 }
 
+// Geometries of one batch that lie on one lattice (equal dx, dt and a, and num_r of one
+// parity, so x_j = (j - num_r/2) dx up to rounding) read g, V' and 1/beta from one shared set
+// of tables, the widest geometry's: a batch of many grid sizes then streams one geometry's
+// tables instead of one per member.  Geometry i of the group starts off = (N - n_i)/2 points
+// into the widest one (N points); it reads the alignment shifted by (-off) mod 8 points, at a
+// chunk offset of (off + shift)/8.  Its first H chunks keep their own pivots (the pivots of
+// a grid that starts later differ from the widest grid's near that start; the difference
+// decays by ~rho^2 per point, below 1e-30 after H chunks).
+int share_lattices(const std::vector<ctdse_geometry *> &gs)
+{
+    std::map<std::tuple<double, double, double, int>, std::vector<ctdse_geometry *>> groups;
+    for (ctdse_geometry *G : gs)
+        if (G && G->dev.g) groups[std::make_tuple(G->dx, G->dt, G->a, G->num_r & 1)].push_back(G);
+    for (auto &kv : groups) {
+        const std::vector<ctdse_geometry *> &grp = kv.second;
+        if (grp.size() < 2) continue;
+        const ctdse_geometry *big = *std::max_element(
+            grp.begin(), grp.end(), [](const ctdse_geometry *p, const ctdse_geometry *q) { return p->num_r < q->num_r; });
+        auto L = std::make_shared<Lattice>();
+        const long long npad = tdse_tiled_size(big->dev.nchunks + 64);
+        for (int s = 0; s < Lattice::kAlign; s++) {
+            HIP_CHECK(L->g[s].alloc(npad * sizeof(double)));
+            HIP_CHECK(L->dpot[s].alloc(npad * sizeof(double)));
+            HIP_CHECK(L->ib[s].alloc(npad * sizeof(double2)));
+            hipLaunchKernelGGL(tdse_lattice_kernel, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, 0,
+                               big->g.as<double>(), big->dpot.as<double>(), big->ib.as<double2>(), big->dev.nx, s,
+                               npad, L->g[s].as<double>(), L->dpot[s].as<double>(), L->ib[s].as<double2>());
+            // chunk maps of the shared chunks that hold lattice rows (chunk k at [k + 64])
+            const int nch = big->dev.nchunks + 1;
+            HIP_CHECK(L->chq[s].alloc((nch + 128) * sizeof(double2)));
+            HIP_CHECK(L->chg[s].alloc((nch + 128) * sizeof(double2)));
+            hipLaunchKernelGGL(tdse_lattice_chunk_kernel, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, 0,
+                               L->g[s].as<double>(), L->ib[s].as<double2>(), nch, L->chq[s].as<double2>(),
+                               L->chg[s].as<double2>());
+        }
+        HIP_CHECK(hipGetLastError());
+        for (ctdse_geometry *G : grp) {
+            const int off = (big->num_r - G->num_r) / 2;
+            const int shift = (Lattice::kAlign - off % Lattice::kAlign) % Lattice::kAlign;
+            G->lattice = L;
+            G->dev.sg = L->g[shift].as<double>();
+            G->dev.sdp = L->dpot[shift].as<double>();
+            G->dev.sib = L->ib[shift].as<double2>();
+            G->dev.schq = L->chq[shift].as<double2>();
+            G->dev.schg = L->chg[shift].as<double2>();
+            G->dev.s_off = (off + shift) / TDSE_CHUNK;
+            G->dev.head_chunks = G->dev.halo_lanes;
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(nullptr));
+    return CTDSE_OK;
+}
+
 // Builds geometries: ground states of all that need one in grouped launches, then the
 // tables.  out[i] is null where status[i] is a hard failure; a ground state that did not
 // converge (CTDSE_ERR_GS_NOT_CONVERGED) still yields a geometry, as in the reference, which
@@ -500,6 +564,14 @@ int geometries_from(const std::vector<GeomSpec> &spec, std::vector<ctdse_geometr
             delete out[i];
             out[i] = nullptr;
         }
+    }
+    if (n > 1 && share_lattices(out) != CTDSE_OK) {
+        for (int i = 0; i < n; i++) {
+            delete out[i];
+            out[i] = nullptr;
+            status[i] = g_status;
+        }
+        return g_status;
     }
     if (first != CTDSE_OK) set_status(first, first_msg);
     else clear_status();

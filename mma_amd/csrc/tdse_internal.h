@@ -59,10 +59,20 @@ struct TdseGeom {
     long long npad;       // tiled array length, tdse_tiled_size(nchunks)
     int halo_lanes;       // H: truncation depth in chunks at the right end of a workgroup segment
     int pint_lo, pint_hi; // PopInt window [k1, k4] (prop.c:311-318)
-    int p0_lo, p0_hi;     // rows where |psi0| > 1e-30 max|psi0|
+    int p0_lo, p0_hi;     // rows where |psi0| > 1e-16 max|psi0| (the terms of <psi0|psi>)
     double dx, dt, xmax;  // x_j = j dx - xmax
     double k36;           // 36 dt / dx^2
     double pint0;         // PopInt at t = tmin (prop.c:86-93)
+    // Lattice tables shared by the geometries of a batch that differ only in their extent
+    // (same dx, dt, a, parity of num_r: x_j = (j - num_r/2) dx on one lattice), or null: g,
+    // V' and 1/beta of the widest geometry in tiled order, where chunk c of this geometry is
+    // shared chunk c + s_off.  The forward kernels read g and V' there, and 1/beta from chunk
+    // head_chunks on (before it, the pivots still feel this geometry's left wall and come from
+    // ib); rows past this grid read as zero.  The per-geometry tables above stay complete.
+    const double *sg, *sdp;
+    const double2 *sib;
+    const double2 *schq, *schg;  // the lattice's chunk transfer maps at shared chunk k + 64
+    int s_off, head_chunks;
 };
 
 // One instance of a batch.
@@ -138,6 +148,10 @@ __global__ void tdse_ground_state_kernel(GsJob *jobs, int n_jobs);
 __global__ void tdse_tables_grid_kernel(TableJob *job);
 __global__ void tdse_tables_pivot_kernel(TableJob *job);
 __global__ void tdse_tables_chunk_kernel(TableJob *job);
+__global__ void tdse_lattice_kernel(const double *g, const double *dpot, const double2 *ib, int nx, int shift,
+                                    long long npad, double *sg, double *sdp, double2 *sib);
+__global__ void tdse_lattice_chunk_kernel(const double *sg, const double2 *sib, int nchunks, double2 *schq,
+                                          double2 *schg);
 __global__ void tdse_tables_ib_kernel(TableJob *jobs);
 // host forms of the serial setup steps (tdse_setup.hip), bit-identical to the kernels
 void tdse_ground_state_host(GsJob &J);

@@ -90,6 +90,36 @@ __device__ inline void load_tables(const TdseGeom &G, int cs, int wave, int lane
     const int c = cs + gl;
     const int cl = min(c, G.nchunks);
     const long long base = tdse_chunk_base(cl);
+    if (G.sg) {
+        // lattice tables shared with the other geometries of the batch (uniform branch): rows
+        // past this grid, and the point left of it, read as zero; the first H chunks and the
+        // last one keep their own pivots and chunk maps (the walls of this grid)
+        const long long sb = tdse_chunk_base(cl + G.s_off);
+        const bool head = c < G.head_chunks;
+        const bool own_maps = head || c >= G.nchunks - 1;
+        T.Qc = ld2(own_maps ? G.chq : G.schq, (own_maps ? cl : cl + G.s_off) + 64);
+        T.Gc = ld2(own_maps ? G.chg : G.schg, (own_maps ? cl : cl + G.s_off) + 64);
+        const double2 zero2 = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < C; i++) {
+            const bool live = c * C + i < G.nx;
+            const double gv = ld1(G.sg, sb + 64 * i), dv = ld1(G.sdp, sb + 64 * i);
+            T.Gv[i] = live ? gv : 0.0;
+            T.dp[i] = live ? dv : 0.0;
+            if constexpr (IB_LDS) {
+                T.IB[i] = stash[i * 64 * nw + gl];
+            } else {
+                const double2 v = ld2(head ? G.ib : G.sib, (head ? base : sb) + 64 * i);
+                T.IB[i] = live ? v : zero2;
+            }
+        }
+        const int jl = c * C - 1, jr = (c + 1) * C;  // the points left and right of the chunk
+        T.gL = lane == 0 && jl >= 0 && jl < G.nx ? ld1(G.sg, tdse_chunk_base(c - 1 + G.s_off) + 64 * (C - 1)) : 0.0;
+        T.gR = lane == 63 && gl < nwl * 64 - 1 && jr < G.nx ? ld1(G.sg, tdse_chunk_base(c + 1 + G.s_off)) : 0.0;
+        return;
+    }
+    T.Qc = ld2(G.chq, cl + 64);
+    T.Gc = ld2(G.chg, cl + 64);
 #pragma unroll
     for (int i = 0; i < C; i++) {
         T.Gv[i] = ld1(G.g, base + 64 * i);
@@ -99,8 +129,6 @@ __device__ inline void load_tables(const TdseGeom &G, int cs, int wave, int lane
             T.IB[i] = ld2(G.ib, base + 64 * i);
         T.dp[i] = ld1(G.dpot, base + 64 * i);
     }
-    T.Qc = ld2(G.chq, cl + 64);
-    T.Gc = ld2(G.chg, cl + 64);
     T.gL = lane == 0 ? ld1(G.g, tdse_chunk_base(min(c - 1, G.nchunks)) + 64 * (C - 1)) : 0.0;  // chunk -1: zero
     T.gR = lane == 63 && gl < nwl * 64 - 1 ? ld1(G.g, tdse_chunk_base(min(c + 1, G.nchunks))) : 0.0;
 }

@@ -279,10 +279,16 @@ __host__ __device__ void pivot_scan(TableJob &J)
     J.pint0 = w;
     double mx = 0;
     for (int j = 0; j < nx; j++) mx = fmax(mx, J.psi0_nat[2 * j] * J.psi0_nat[2 * j] + J.psi0_nat[2 * j + 1] * J.psi0_nat[2 * j + 1]);
+    // the rows of <psi0|psi> (PopTot): |psi0_j| above 1e-16 max|psi0|.  The rows left out add at
+    // most 1e-16 max|psi0| sqrt(nx) |psi|_2 <= 4e-15 to <psi0|psi> (Cauchy-Schwarz; |psi|_2 = 1,
+    // max|psi0| < 0.25, nx <= 32001), the rounding level of the sum itself.  Far from the atom
+    // the inverse iteration leaves |psi0| on a plateau (near 5e-17 max after the 4 iterations of
+    // CV = 1e-12, 5e-25 after the 6 of CV = 1e-25): a window that kept the plateau would cover
+    // the whole grid and make every step read psi0 at every point.
     int lo = nx, hi = -1;
     for (int j = 0; j < nx; j++) {
         const double v = J.psi0_nat[2 * j] * J.psi0_nat[2 * j] + J.psi0_nat[2 * j + 1] * J.psi0_nat[2 * j + 1];
-        if (v > 1e-60 * mx) {
+        if (v > 1e-32 * mx) {
             if (j < lo) lo = j;
             hi = j;
         }
@@ -343,6 +349,52 @@ __global__ void tdse_tables_chunk_kernel(TableJob *jobs)
         J.chg[c + 64] = Hh;
         J.adq[c + 64] = AQ;
         J.adg[c + 64] = AG;
+    }
+}
+
+// One alignment of a shared lattice table set (TdseGeom::sg): shared tiled index t holds
+// point t_point - shift of the widest geometry (g, dpot, ib tiled, nx points), zero outside
+// [0, nx).  One thread per tiled index; npad: the shared arrays' length.
+__global__ void tdse_lattice_kernel(const double *g, const double *dpot, const double2 *ib, int nx, int shift,
+                                    long long npad, double *sg, double *sdp, double2 *sib)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < npad;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long tile = t / TDSE_TILE - 1, rem = t % TDSE_TILE;
+        const long long j = (tile * 64 + (rem % 64)) * TDSE_CHUNK + rem / 64 - shift;
+        if (tile >= 0 && j >= 0 && j < nx) {
+            const long long src = tdse_index(j);
+            sg[t] = g[src];
+            sdp[t] = dpot[src];
+            sib[t] = ib[src];
+        } else {
+            sg[t] = 0.0;
+            sdp[t] = 0.0;
+            sib[t] = make_double2(0.0, 0.0);
+        }
+    }
+}
+
+// The chunk transfer maps of one lattice alignment (as tdse_tables_chunk_kernel, from the
+// shared tiled g and 1/beta; rows outside the lattice are zero there): forward product of
+// q_j = -(1 + i g_{j-1}) ib_j and backward product of h_j = -(1 + i g_{j+1}) ib_j over each
+// chunk, chunk k at [k + 64].  One thread per chunk, nchunks chunks.
+__global__ void tdse_lattice_chunk_kernel(const double *sg, const double2 *sib, int nchunks, double2 *schq,
+                                          double2 *schg)
+{
+    auto mul = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+    auto coef = [](double g, double2 ib) { return make_double2(-(ib.x - g * ib.y), -(ib.y + g * ib.x)); };
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
+        double2 Q = make_double2(1., 0.), Hh = make_double2(1., 0.);
+        for (int i = 0; i < TDSE_CHUNK; i++) {
+            const long long j = (long long)c * TDSE_CHUNK + i;
+            const double2 ib = sib[tdse_index(j)];
+            const double gm = j > 0 ? sg[tdse_index(j - 1)] : 0.0;  // point -1: the zero row before
+            Q = mul(Q, coef(gm, ib));
+            Hh = mul(Hh, coef(sg[tdse_index(j + 1)], ib));
+        }
+        schq[c + 64] = Q;
+        schg[c + 64] = Hh;
     }
 }
 

@@ -33,6 +33,39 @@ def test_create_many_equals_one_at_a_time():
         np.testing.assert_array_equal(pa, pb)
 
 
+def test_shared_lattice_tables_match_own_tables():
+    """Geometries built together on one lattice (equal dx, dt, a and num_r parity) read g, V'
+    and 1/beta of the widest one, each from the alignment that puts its chunks on shared chunk
+    boundaries, with its own pivots in its first H chunks; the results equal those of the same
+    geometries built one at a time (own tables) to rounding.  Offsets 0..7 into the widest even
+    grid (every alignment), an odd group, one and two steps per pass."""
+    import os
+    even = [3000 - 2 * k for k in range(8)]
+    odd = [2001, 1999, 1501]
+    sizes = even + odd
+    shared = tdse.Geometry.create_many(sizes, 0.1, 0.25, W.GRID["a"], W.GRID["e_guess"], 1e-25, 2.0)
+    own = [_grid_geometry(n) for n in sizes]
+    rng = np.random.default_rng(21)
+    t = np.arange(420) * 0.25
+    f = np.stack([W.sin2_pulse(t, rng.uniform(0.04, 0.09), W.omega_au(800e-9), 3, phase=rng.uniform(0, 3))
+                  for _ in sizes])
+    dev = torch.tensor(tdse.pad_fields(f), device="cuda")
+    steps = [dev.shape[1] - 1 - (i % 3) for i in range(len(sizes))]
+    for spp in ("1", "2"):
+        os.environ["MMA_TDSE_STEPS_PER_PASS"] = spp
+        try:
+            a = tdse.propagate_batch(shared, dev, n_steps=steps).cpu().numpy()
+            b = tdse.propagate_batch(own, dev, n_steps=steps).cpu().numpy()
+        finally:
+            del os.environ["MMA_TDSE_STEPS_PER_PASS"]
+        for i, n in enumerate(sizes):
+            for r, name in enumerate(ROWS[:4]):
+                e = rel_err(a[i, r, :steps[i]], b[i, r, :steps[i]])
+                assert e < 1e-11, (spp, n, name, e)
+        print("steps/pass", spp, "shared vs own tables: max %.2e" % max(
+            rel_err(a[i, 0, :steps[i]], b[i, 0, :steps[i]]) for i in range(len(sizes))), flush=True)
+
+
 def test_create_many_reports_bad_members():
     with pytest.raises(tdse.CTDSEError):
         tdse.Geometry.create_many([1000, 1], 0.1, 0.25, W.GRID["a"])
@@ -122,28 +155,34 @@ def test_autotune_returns_a_timed_candidate(tmp_path, monkeypatch):
 
 def test_uneven_workload_members_match_oracle():
     """Members of the seeded uneven workload (own Nx, cycles, intensity, wavelength) in one
-    launch through a tuned plan; the smallest are checked against the oracle and every
-    member conserves the norm."""
+    launch through a plan with the shape the autotuner picks for the whole workload (4 waves x
+    2 workgroups per CU, two steps per pass) and the batch's shared lattice tables: the three
+    smallest against the oracle over their whole run, the widest (Nx 31997) and the one with the
+    most work (Nx 31723, 19477 steps) over their first 1500 steps, and every member conserves the
+    norm."""
     from oracle import ctdse as O
     num_r, n_e = W.uneven_work()
     work = (num_r + 1) * (n_e + 2)
-    pick = [int(i) for i in np.argsort(work)[:12]] + [int(np.argmax(work))]
+    pick = [int(i) for i in np.argsort(work)[:12]] + [int(np.argmax(num_r)), int(np.argmax(work))]
     sel = W.uneven_batch(indices=pick)
     geoms = tdse.Geometry.create_many([m["num_r"] for m in sel], 0.1, 0.25, W.GRID["a"], W.GRID["e_guess"],
                                       W.UNEVEN_CV, 2.0)
     fields, steps = W.stack_fields([m["field"] for m in sel])
     dev = torch.tensor(fields, device="cuda")
-    plan = tdse.BatchPlan(geoms, dev, n_steps=steps)
+    plan = tdse.BatchPlan(geoms, dev, n_steps=steps, config=(4, 2, 2))
     out = plan.replay().cpu().numpy()
     for i, s in enumerate(steps):  # instance i owns columns [0, steps[i])
         assert np.all(out[i, 1, :s] <= 1.0 + 1e-12) and np.all(np.isfinite(out[i, :, :s])), pick[i]
-    for i in range(3):
-        m, g, s = sel[i], geoms[i], steps[i]
+    widest, longest = len(sel) - 2, len(sel) - 1
+    assert sel[widest]["num_r"] + 1 == 31997 and steps[longest] == 19477
+    for i, window in [(0, None), (1, None), (2, None), (widest, 1500), (longest, 1500)]:
+        m, g = sel[i], geoms[i]
+        s = steps[i] if window is None else window
         x, psi0 = g.ground_state()
         ref = O.propagate(m["num_r"], 0.1, W.GRID["a"], 2.0, 0.25, psi0, x, fields[i], s)
         for r, k in enumerate(ROWS):
             e = rel_err(out[i, r, :s], ref[k][:s])
-            print(pick[i], m["num_r"], k, "%.2e" % e, flush=True)
+            print(pick[i], m["num_r"], s, k, "%.2e" % e, flush=True)
             assert e < TOL, (pick[i], k)
     plan.close()
 
