@@ -11,7 +11,10 @@ import os
 __version__ = "0.1.0"
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libsingleTDSE.so")
+# MMA_AMD_LIB_DIR: load the native libraries from another directory (e.g. a host-sanitizer
+# build made by ``python -m mma_amd.build --sanitize DIR``, which puts them in DIR/lib)
+LIB_DIR = os.environ.get("MMA_AMD_LIB_DIR") or os.path.join(HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libsingleTDSE.so")
 
 
 class NativeLibraryMissing(RuntimeError):

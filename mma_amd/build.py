@@ -3,6 +3,10 @@ mma_amd/lib/libmmahankel.so (Hankel far-field transform) and mma_amd/lib/libmmap
 benchmark's bandwidth probes, loaded by bench.py only).
 
 Usage: python -m mma_amd.build   (or mma_amd.build.build() from Python)
+       python -m mma_amd.build --usage            (also print each kernel's registers and spills)
+       python -m mma_amd.build --sanitize DIR     (host code with AddressSanitizer and
+           UndefinedBehaviorSanitizer, device code unchanged, into DIR/lib; load it with
+           MMA_AMD_LIB_DIR=DIR/lib and clang's ASan runtime preloaded)
 """
 import os
 import subprocess
@@ -52,22 +56,35 @@ def _run(cmd, verbose):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=False, resource_usage=False):
-    os.makedirs(BUILD, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+# host-only sanitizers: each -fsanitize= applies to the host compilation alone
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def build(verbose=False, resource_usage=False, sanitize=None):
+    """Compile every library; with `sanitize` (a directory) build the host-sanitized variant
+    there (objects in sanitize/_build, libraries in sanitize/lib) instead."""
+    build_dir, lib_dir, extra_all, link_extra = BUILD, LIBDIR, [], []
+    if sanitize:
+        build_dir, lib_dir = os.path.join(sanitize, "_build"), os.path.join(sanitize, "lib")
+        extra_all, link_extra = SANITIZE, SANITIZE + ["-shared-libsan"]
+    os.makedirs(build_dir, exist_ok=True)
+    os.makedirs(lib_dir, exist_ok=True)
     objs = {}
     for src, flags in SOURCES:
-        obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
+        obj = os.path.join(build_dir, src.rsplit(".", 1)[0] + ".o")
         extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage and src.endswith(".hip") else []
-        _run([HIPCC] + COMMON + flags + extra + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        _run([HIPCC] + COMMON + flags + extra_all + extra + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs[src] = obj
     for lib, srcs in LIBS.items():
-        out = os.path.join(LIBDIR, lib)
+        out = os.path.join(lib_dir, lib)
         tmp = out + ".tmp"
-        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [objs[s] for s in srcs], verbose)
+        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC"] + link_extra + ["-o", tmp] +
+             [objs[s] for s in srcs], verbose)
         os.replace(tmp, out)
-    return os.path.join(LIBDIR, "libsingleTDSE.so")
+    return os.path.join(lib_dir, "libsingleTDSE.so")
 
 
 if __name__ == "__main__":
-    print(build(verbose=True, resource_usage="--usage" in sys.argv))
+    san = sys.argv[sys.argv.index("--sanitize") + 1] if "--sanitize" in sys.argv else None
+    print(build(verbose=True, resource_usage="--usage" in sys.argv, sanitize=san))

@@ -26,9 +26,9 @@ from ctypes import POINTER, Structure, byref, c_double, c_int, c_longlong, c_voi
 import numpy as np
 import torch
 
-from . import HERE, NativeLibraryMissing
+from . import HERE, LIB_DIR, NativeLibraryMissing
 
-LIB_PATH = os.path.join(HERE, "lib", "libmmahankel.so")
+LIB_PATH = os.path.join(LIB_DIR, "libmmahankel.so")
 TABLES = os.path.join(HERE, "data", "xuv_tables.npz")
 C_LIGHT = 299792458.0  # shared_python/units.py:3
 

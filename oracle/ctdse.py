@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libctdse_oracle.so")
+# ORACLE_LIB_DIR: an oracle build from elsewhere (the host-sanitizer run builds one with ASan)
+LIB_PATH = os.path.join(os.environ.get("ORACLE_LIB_DIR") or os.path.join(HERE, "_build"), "libctdse_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_ctdse.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
