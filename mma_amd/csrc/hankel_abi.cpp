@@ -5,6 +5,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "hankel_internal.h"
@@ -103,6 +104,20 @@ int source_q_max(const hk_planes *p, const HostGrids &g, double *q_max)
     return HK_OK;
 }
 
+// r_j = r_0 + j dr to within a few ulps of max |r| (then angles advanced by rotation stay within
+// ~1e-13 rad of the exact ones)
+bool uniform_radii(const std::vector<double> &r, double *dr_out)
+{
+    const double dr = (r.back() - r.front()) / (double)(r.size() - 1);
+    double rmax = 0.0;
+    for (double v : r) rmax = std::max(rmax, std::fabs(v));
+    bool uniform = dr > 0;
+    for (size_t j = 1; j < r.size() && uniform; j++)
+        uniform = std::fabs(r[j] - (r.front() + (double)j * dr)) <= 8.0 * 2.220446049250313e-16 * rmax;
+    *dr_out = dr;
+    return uniform;
+}
+
 } // namespace
 
 extern "C" {
@@ -184,13 +199,8 @@ int hk_direct(const hk_planes *p, int n_out, const double *weights, int per_plan
     // a uniform radial grid lets the kernel rotate cos/sin of the Bessel argument
     std::vector<double> r(p->nr);
     HK_HIP(hipMemcpy(r.data(), p->rgrid, sizeof(double) * p->nr, hipMemcpyDeviceToHost));
-    // (every r_j within a few ulps of r_0 + j dr, so rotated angles stay within ~1e-13 rad)
-    const double dr = (r.back() - r.front()) / (p->nr - 1);
-    double rmax = 0.0;
-    for (double v : r) rmax = std::max(rmax, std::fabs(v));
-    bool uniform = dr > 0;
-    for (int j = 1; j < p->nr && uniform; j++)
-        uniform = std::fabs(r[j] - (r.front() + j * dr)) <= 8.0 * 2.220446049250313e-16 * rmax;
+    double dr = 0.0;
+    const bool uniform = uniform_radii(r, &dr);
     hipStream_t s = (hipStream_t)stream;
     dim3 grid(p->no, (p->nrho + 255) / 256);
     if (p->src_c64)
@@ -241,8 +251,10 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
     if (q_max / h > (double)(nq - HK_GRID_P - 1) * (1.0 + 1e-9))
         return fail(HK_ERR_ARGUMENT, "q grid too short for these sources (use hk_grid_size)");
     hipStream_t s = (hipStream_t)stream;
-    // bins, then (when they fit the LDS too) 5 doubles of per-plane factors
-    const size_t lds_bins = (size_t)nq * sizeof(double2), lds_tab = lds_bins + (size_t)p->nz * 5 * sizeof(double);
+    if (type != HK_F64 && type != HK_F32) return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");
+    // bins, then (when they fit the LDS too) the per-plane factors of the register-window kernel
+    const size_t lds_bins = (size_t)nq * sizeof(double2);
+    const size_t lds_tab = lds_bins + (size_t)p->nz * kSpreadTab * sizeof(double);
     const bool tab = lds_tab <= (size_t)kMaxBins * sizeof(double2);
     const size_t lds = tab ? lds_tab : lds_bins;
     auto launch = [&](auto kernel) -> int {
@@ -251,12 +263,28 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
         return HK_OK;
     };
     int st;
-    if (type == HK_F64)
-        st = tab ? launch(hk_spread_kernel<HK_F64, true>) : launch(hk_spread_kernel<HK_F64, false>);
-    else if (type == HK_F32)
-        st = tab ? launch(hk_spread_kernel<HK_F32, true>) : launch(hk_spread_kernel<HK_F32, false>);
-    else
-        return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");
+    // near-field phase K r^2 / (2 D): the reduced-range sincos covers it when below kSincosMax
+    const bool c64 = p->src_c64 != 0;
+    double r_max = 0.0, o_max = 0.0, d_min = INFINITY;
+    for (double v : g.r) r_max = std::max(r_max, std::fabs(v));
+    for (double v : g.o) o_max = std::max(o_max, std::fabs(v));
+    for (double v : g.z) d_min = std::min(d_min, p->distance - v);
+    const int near = !p->near_field ? 0 : (o_max * kInvC * r_max * r_max / (2.0 * d_min) < 0.5 * kSincosLimit ? 1 : 2);
+    if (tab) {
+        auto pick = [&](auto type_tag, auto src_tag) -> int {
+            constexpr int T = decltype(type_tag)::value;
+            using S = typename decltype(src_tag)::type;
+            return near == 0 ? launch(hk_spread_window_kernel<T, S, 0>)
+                   : near == 1 ? launch(hk_spread_window_kernel<T, S, 1>)
+                               : launch(hk_spread_window_kernel<T, S, 2>);
+        };
+        using F64 = std::integral_constant<int, HK_F64>;
+        using F32 = std::integral_constant<int, HK_F32>;
+        struct D2 { using type = double2; };
+        struct F2 { using type = float2; };
+        st = type == HK_F64 ? (c64 ? pick(F64{}, F2{}) : pick(F64{}, D2{})) : (c64 ? pick(F32{}, F2{}) : pick(F32{}, D2{}));
+    } else
+        st = type == HK_F64 ? launch(hk_spread_kernel<HK_F64>) : launch(hk_spread_kernel<HK_F32>);
     if (st != HK_OK) return st;
     HK_HIP(hipGetLastError());
     HK_HIP(hipStreamSynchronize(s));

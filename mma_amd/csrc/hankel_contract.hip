@@ -40,7 +40,7 @@ __device__ inline double2 source_term(const hk_planes &p, const S *plane, int k,
 }
 
 constexpr int kThreads = 256;
-constexpr int kSpreadThreads = 1024;  // hk_spread_kernel's workgroup
+constexpr int kSpreadThreads = 1024;  // the spread kernels' workgroup
 constexpr int kJTile = 512;     // radial points staged in LDS per pass
 constexpr int kAnchor = 32;     // exact sincos every kAnchor rotated steps
 
@@ -133,6 +133,47 @@ template __global__ void hk_direct_kernel<float2>(hk_planes, int, const double *
 
 namespace {
 
+// sin and cos of x for |x| < kSincosLimit: Cody-Waite reduction x = n pi/2 + y with pi/2 split into
+// three doubles (each product n * c_i exact inside an FMA), then the Taylor series of sin y
+// (through y^17) and cos y (through y^18) on |y| <= pi/4, whose first omitted terms are below
+// 1e-19, and the quadrant n mod 4.  The constants are pi/2 = c1 + c2 + c3 and 1/k! rounded to
+// double.  Fewer registers and instructions than the general routine, which also serves
+// arguments up to 1e308.
+__device__ inline void sincos_reduced(double x, double *s, double *c)
+{
+    constexpr double c1 = 0x1.921fb54442d18p+0, c2 = 0x1.1a62633145c07p-54, c3 = -0x1.f1976b7ed8fbcp-110;
+    constexpr double two_over_pi = 0x1.45f306dc9c883p-1;
+    const double n = rint(x * two_over_pi);
+    double y = fma(-n, c1, x);
+    y = fma(-n, c2, y);
+    y = fma(-n, c3, y);
+    const double z = y * y;
+    // 1/k! with alternating signs
+    double ps = 2.8114572543455206e-15;
+    ps = fma(ps, z, -7.647163731819816e-13);
+    ps = fma(ps, z, 1.6059043836821613e-10);
+    ps = fma(ps, z, -2.505210838544172e-08);
+    ps = fma(ps, z, 2.7557319223985893e-06);
+    ps = fma(ps, z, -0.0001984126984126984);
+    ps = fma(ps, z, 0.008333333333333333);
+    ps = fma(ps, z, -0.16666666666666666);
+    const double sy = fma(y * z, ps, y);
+    double pc = -1.5619206968586225e-16;
+    pc = fma(pc, z, 4.779477332387385e-14);
+    pc = fma(pc, z, -1.1470745597729725e-11);
+    pc = fma(pc, z, 2.08767569878681e-09);
+    pc = fma(pc, z, -2.755731922398589e-07);
+    pc = fma(pc, z, 2.48015873015873e-05);
+    pc = fma(pc, z, -0.001388888888888889);
+    pc = fma(pc, z, 0.041666666666666664);
+    pc = fma(pc, z, -0.5);
+    const double cy = fma(z, pc, 1.0);
+    const int q = (int)n & 3;
+    const double sa = (q & 1) ? cy : sy, ca = (q & 1) ? sy : cy;
+    *s = (q & 2) ? -sa : sa;
+    *c = ((q + 1) & 2) ? -ca : ca;
+}
+
 // Lagrange basis of HK_GRID_P nodes at offsets d_i = i - (P/2 - 1) from floor(t), at f = frac(t)
 struct Lagrange {
     double L[HK_GRID_P];
@@ -171,19 +212,204 @@ __device__ inline void store_operand(void *out, long long at, double v)
         reinterpret_cast<float *>(out)[at] = (float)v;
 }
 
+// Adds the Lagrange-weighted item v at grid position t to the bins (real and imaginary parts
+// in separate LDS arrays: fewer bank conflicts than interleaved pairs).
+__device__ inline void spread_item(double *acc_re, double *acc_im, int nq, double t, double2 v)
+{
+    const double b = floor(t);
+    const Lagrange lw(t - b);
+    const int l0 = (int)b + 1;  // bin of node offset -(P/2 - 1): Q_l = (l - P/2) h
+    if (l0 < 0 || l0 + HK_GRID_P > nq) return;  // the host sizes the grid so this never holds
+#pragma unroll
+    for (int i = 0; i < HK_GRID_P; i++) {
+        atomicAdd(&acc_re[l0 + i], lw.L[i] * v.x);
+        atomicAdd(&acc_im[l0 + i], lw.L[i] * v.y);
+    }
+}
+
+template <int TYPE>
+__device__ inline void store_bins(const double *acc_re, const double *acc_im, int nq, void *out, long long ld,
+                                  long long re_row)
+{
+    for (int l = threadIdx.x; l < nq; l += blockDim.x) {
+        store_operand<TYPE>(out, re_row * ld + l, acc_re[l]);
+        store_operand<TYPE>(out, (re_row + 1) * ld + l, acc_im[l]);
+    }
+}
+
 } // namespace
 
 // one workgroup per (frequency, output); the nq bins of C(., omega) accumulate in LDS (FP64
-// atomics, real and imaginary parts in separate arrays: fewer bank conflicts than interleaved
-// pairs).
-// The (plane, radius) pairs of the planes with non-zero weight are dealt to the 1024 threads
-// in one flat range, advanced with 32-bit counters, so every wave has work whatever nr is.
-// TAB: the per-plane factors (K / D_k / h, -K / (2 D_k), w_k, pre_k) sit in LDS after the
-// bins (5 doubles per plane), so an item does no division and no pre-factor load.
-template <int TYPE, bool TAB>
+// atomics).  The (plane, radius) pairs of the planes with non-zero weight are dealt to the 1024
+// threads in one flat range, advanced with 32-bit counters, so every wave has work whatever nr
+// is.  The general form: per-plane factors recomputed per item (used when hk_spread_window_kernel's
+// table of them does not fit the LDS beside the bins).
+template <int TYPE>
 __global__ void __launch_bounds__(kSpreadThreads)
 hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double inv_c, void *out, long long ld,
                  long long row0)
+{
+    extern __shared__ double bins[];
+    __shared__ int k_range[2];
+    double *acc_re = bins, *acc_im = bins + nq;
+    const int o = blockIdx.x;
+    weights += (long long)blockIdx.y * p.nz;  // output blockIdx.y
+    row0 += 2LL * p.no * blockIdx.y;
+    const double K = p.ogrid[o] * inv_c;
+    const double inv_h = 1.0 / h;
+    for (int l = threadIdx.x; l < 2 * nq; l += blockDim.x) bins[l] = 0.0;
+    if (threadIdx.x == 0) {
+        k_range[0] = p.nz;
+        k_range[1] = -1;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < p.nz; k += blockDim.x) {
+        if (weights[k] != 0.0) {
+            atomicMin(&k_range[0], k);
+            atomicMax(&k_range[1], k);
+        }
+    }
+    __syncthreads();
+    const int k_end = k_range[1] + 1;
+    const bool c64 = p.src_c64 != 0;
+    const int step_k = blockDim.x / p.nr, step_j = blockDim.x - step_k * p.nr;
+    int k = k_range[0] + threadIdx.x / p.nr, j = threadIdx.x % p.nr;
+    for (; k < k_end; k += step_k, j += step_j) {
+        if (j >= p.nr) {  // step_j < nr: one wrap at most
+            j -= p.nr;
+            if (++k >= k_end) break;
+        }
+        const double wk = weights[k];
+        if (wk == 0.0) continue;
+        const double D = p.distance - p.zgrid[k];
+        double2 pk = make_double2(1.0, 0.0);
+        if (p.pre_mode == 1) pk = reinterpret_cast<const double2 *>(p.pre)[(long long)k * p.no + o];
+        double2 v = c64 ? source_term(p, reinterpret_cast<const float2 *>(p.src) + (long long)k * p.plane_stride, k, o,
+                                      j, K, D, pk)
+                        : source_term(p, reinterpret_cast<const double2 *>(p.src) + (long long)k * p.plane_stride, k,
+                                      o, j, K, D, pk);
+        v.x *= wk;
+        v.y *= wk;
+        spread_item(acc_re, acc_im, nq, (K / D) * p.rgrid[j] * inv_h, v);
+    }
+    __syncthreads();
+    store_bins<TYPE>(acc_re, acc_im, nq, out, ld, row0 + 2LL * o);
+}
+
+template __global__ void hk_spread_kernel<HK_F64>(hk_planes, const double *, int, double, double, void *, long long,
+                                                  long long);
+template __global__ void hk_spread_kernel<HK_F32>(hk_planes, const double *, int, double, double, void *, long long,
+                                                  long long);
+
+// The same spread, accumulated in registers.  A thread owns one radius r_j (and a contiguous
+// range of planes) and walks its planes in order.  Its items' bins l0 = floor(K r_j / (D_k h)) + 1
+// hardly move from plane to plane (D_k changes little), so the thread keeps the 8 bins
+// l0 .. l0 + 7 of its current stencil in registers, adds each item's 8 Lagrange-weighted terms
+// there, and writes a bin to the LDS (FP64 atomic) only when the stencil moves past it: for
+// ascending planes a radius flushes (its bin span over the planes + 8) bins, instead of 8 per
+// item.  Per-plane factors (K / D / h, -K / (2 D), weight, pre-factor) sit in LDS after the
+// bins (kSpreadTab doubles per plane).  Any plane order is handled (a backward or far jump
+// flushes the whole stencil).
+struct Window {
+    double re[HK_GRID_P], im[HK_GRID_P];
+    int l0;  // bin of re[0]; below 0: empty
+};
+
+__device__ inline void window_flush_all(Window &w, double *acc_re, double *acc_im)
+{
+#pragma unroll
+    for (int m = 0; m < HK_GRID_P; m++) {
+        if (w.re[m] != 0.0) atomicAdd(&acc_re[w.l0 + m], w.re[m]);
+        if (w.im[m] != 0.0) atomicAdd(&acc_im[w.l0 + m], w.im[m]);
+        w.re[m] = 0.0;
+        w.im[m] = 0.0;
+    }
+}
+
+// adds v times the Lagrange basis at f (the same weights as Lagrange(f)) to the stencil, each
+// weight formed as it is used: the suffix products are kept, the prefix product runs
+__device__ inline void window_add(Window &w, double f, double2 v)
+{
+    constexpr int P = HK_GRID_P;
+    double suf[P];
+    suf[P - 1] = 1.0;
+#pragma unroll
+    for (int i = P - 2; i >= 0; i--) suf[i] = suf[i + 1] * (f - (double)(i + 1 - (P / 2 - 1)));
+    double pre = 1.0;
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+        double den = 1.0;
+        for (int q = 2; q <= i; q++) den *= (double)q;
+        for (int q = 2; q <= P - 1 - i; q++) den *= (double)q;
+        const double inv = ((P - 1 - i) & 1 ? -1.0 : 1.0) / den;  // folded at compile time
+        const double L = (pre * suf[i]) * inv;
+        w.re[i] = fma(L, v.x, w.re[i]);
+        w.im[i] = fma(L, v.y, w.im[i]);
+        pre *= f - (double)(i - (P / 2 - 1));
+    }
+}
+
+// moves the stencil to start at bin l0 (0 <= l0 <= nq - HK_GRID_P)
+__device__ inline void window_move(Window &w, int l0, double *acc_re, double *acc_im)
+{
+    if (w.l0 < 0) {
+        w.l0 = l0;
+        return;
+    }
+    while (w.l0 != l0) {
+        if (l0 > w.l0 && l0 < w.l0 + HK_GRID_P) {
+            atomicAdd(&acc_re[w.l0], w.re[0]);
+            atomicAdd(&acc_im[w.l0], w.im[0]);
+#pragma unroll
+            for (int m = 0; m < HK_GRID_P - 1; m++) {
+                w.re[m] = w.re[m + 1];
+                w.im[m] = w.im[m + 1];
+            }
+            w.re[HK_GRID_P - 1] = 0.0;
+            w.im[HK_GRID_P - 1] = 0.0;
+            w.l0++;
+        } else {
+            window_flush_all(w, acc_re, acc_im);
+            w.l0 = l0;
+        }
+    }
+}
+
+// A plane's factors for one frequency, read from the LDS table in one go.
+struct PlaneTab {
+    double s, c, w, pre_re, pre_im;
+};
+__device__ inline PlaneTab plane_tab(const double *tab, int k)
+{
+    const double2 *t = reinterpret_cast<const double2 *>(tab + kSpreadTab * k);  // 16-byte aligned
+    const double2 a = t[0], b = t[1], c = t[2];
+    return {a.x, a.y, b.x, b.y, c.x};
+}
+
+// e^{i x} for |x| <= 2^-7: Taylor series through x^7 (sin) and x^8 (cos), first omitted terms
+// below 1e-24
+__device__ inline double2 expi_small(double x)
+{
+    const double z = x * x;
+    double ps = -1.0 / 5040.0;
+    ps = fma(ps, z, 1.0 / 120.0);
+    ps = fma(ps, z, -1.0 / 6.0);
+    double pc = 1.0 / 40320.0;
+    pc = fma(pc, z, -1.0 / 720.0);
+    pc = fma(pc, z, 1.0 / 24.0);
+    pc = fma(pc, z, -0.5);
+    return make_double2(fma(z, pc, 1.0), fma(x * z, ps, x));
+}
+
+constexpr int kPhaseAnchor = 32;  // planes between exact near-field phases
+
+// S: the source element type (double2 or float2), kept as loaded while it is in flight.
+// NEAR: 0 no near-field phase, 1 its sincos by sincos_reduced (|phase| < kSincosLimit / 2,
+// checked by the host), 2 by the general sincos.
+template <int TYPE, typename S, int NEAR>
+__global__ void __launch_bounds__(kSpreadThreads)
+hk_spread_window_kernel(hk_planes p, const double *weights, int nq, double h, double inv_c, void *out, long long ld,
+                        long long row0)
 {
     extern __shared__ double bins[];
     __shared__ int k_range[2];
@@ -205,89 +431,102 @@ hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double in
             atomicMin(&k_range[0], k);
             atomicMax(&k_range[1], k);
         }
-        if constexpr (TAB) {
-            const double D = p.distance - p.zgrid[k];
-            double2 pk = make_double2(1.0, 0.0);
-            if (p.pre_mode == 1) pk = reinterpret_cast<const double2 *>(p.pre)[(long long)k * p.no + o];
-            double *t = tab + 5 * k;
-            t[0] = (K / D) * inv_h;
-            t[1] = -K / (2.0 * D);
-            t[2] = wk;
-            t[3] = pk.x;
-            t[4] = pk.y;
-        }
+        const double D = p.distance - p.zgrid[k];
+        double2 pk = make_double2(1.0, 0.0);
+        if (p.pre_mode == 1) pk = reinterpret_cast<const double2 *>(p.pre)[(long long)k * p.no + o];
+        double *t = tab + kSpreadTab * k;
+        t[0] = (K / D) * inv_h;
+        t[1] = -K / (2.0 * D);
+        t[2] = wk;
+        t[3] = pk.x;
+        t[4] = pk.y;
     }
     __syncthreads();
-    const int k_end = k_range[1] + 1;
-    const bool c64 = p.src_c64 != 0;
-    const int step_k = blockDim.x / p.nr, step_j = blockDim.x - step_k * p.nr;
-    int k = k_range[0] + threadIdx.x / p.nr, j = threadIdx.x % p.nr;
-    for (; k < k_end; k += step_k, j += step_j) {
-        if (j >= p.nr) {  // step_j < nr: one wrap at most
-            j -= p.nr;
-            if (++k >= k_end) break;
-        }
-        double2 v;
-        double t;
-        if constexpr (TAB) {
-            const double *tk = tab + 5 * k;
-            const double wk = tk[2];
-            if (wk == 0.0) continue;
+    const int k_lo = k_range[0], nk = k_range[1] + 1 - k_lo;
+    // thread -> (radius, plane group): a group is a whole number of waves (so a wave's plane is
+    // uniform), several groups of contiguous planes when the radii leave waves over
+    const int span = (p.nr + 63) & ~63;
+    const int groups = span >= (int)blockDim.x ? 1 : min((int)blockDim.x / span, max(nk, 1));
+    const int g = groups == 1 ? 0 : threadIdx.x / span;
+    if (nk > 0 && g < groups) {
+        const int ka = k_lo + (int)((long long)nk * g / groups), kb = k_lo + (int)((long long)nk * (g + 1) / groups);
+        const int first = groups == 1 ? threadIdx.x : threadIdx.x - g * span;
+        const int stride = groups == 1 ? (int)blockDim.x : span;
+        for (int j = first; j < p.nr; j += stride) {
             const double r = p.rgrid[j];
-            const long long at = (long long)k * p.plane_stride + (long long)o * p.nr + j;
-            const double2 f = c64 ? load_src(reinterpret_cast<const float2 *>(p.src), at)
-                                  : load_src(reinterpret_cast<const double2 *>(p.src), at);
-            v = cmul(make_double2(tk[3], tk[4]), f);
-            if (p.near_field) {
-                double sn, cs;
-                sincos(tk[1] * (r * r), &sn, &cs);
-                v = cmul(make_double2(cs, sn), v);
-            }
-            const double wr = (p.tw[j] * r) * wk;
-            v.x *= wr;
-            v.y *= wr;
-            t = tk[0] * r;
-        } else {
-            const double wk = weights[k];
-            if (wk == 0.0) continue;
-            const double D = p.distance - p.zgrid[k];
-            const double s = K / D;
-            double2 pk = make_double2(1.0, 0.0);
-            if (p.pre_mode == 1) pk = reinterpret_cast<const double2 *>(p.pre)[(long long)k * p.no + o];
-            v = c64 ? source_term(p, reinterpret_cast<const float2 *>(p.src) + (long long)k * p.plane_stride, k, o, j,
-                                  K, D, pk)
-                    : source_term(p, reinterpret_cast<const double2 *>(p.src) + (long long)k * p.plane_stride, k, o,
-                                  j, K, D, pk);
-            v.x *= wk;
-            v.y *= wk;
-            t = s * p.rgrid[j] * inv_h;
-        }
-        const double b = floor(t);
-        const Lagrange lw(t - b);
-        const int l0 = (int)b + 1;  // bin of node offset -(P/2 - 1): Q_l = (l - P/2) h
-        if (l0 < 0 || l0 + HK_GRID_P > nq) continue;  // the host sizes the grid so this never holds
+            const double wr = p.tw[j] * r;
+            const S *src = reinterpret_cast<const S *>(p.src) + (long long)ka * p.plane_stride + (long long)o * p.nr + j;
+            Window w;
+            w.l0 = -1;
 #pragma unroll
-        for (int i = 0; i < HK_GRID_P; i++) {
-            atomicAdd(&acc_re[l0 + i], lw.L[i] * v.x);
-            atomicAdd(&acc_im[l0 + i], lw.L[i] * v.y);
+            for (int m = 0; m < HK_GRID_P; m++) {
+                w.re[m] = 0.0;
+                w.im[m] = 0.0;
+            }
+            // the near-field phase e^{i c_k r^2} advances from plane to plane by the small angle
+            // (c_k - c_prev) r^2 (consecutive planes: ~K r^2 dz / (2 D^2)); an exact sincos where
+            // that angle is not small, and every kPhaseAnchor planes
+            double2 ph = make_double2(1.0, 0.0);
+            double c_prev = 0.0;
+            int since = kPhaseAnchor;
+            const double r2 = r * r;
+            S f1 = src[0];
+            for (int k = ka; k < kb; k++) {
+                const S fs = f1;
+                src += p.plane_stride;
+                if (k + 1 < kb) f1 = src[0];  // the next plane's source in flight
+                const PlaneTab tk = plane_tab(tab, k);
+                if (tk.w == 0.0) continue;
+                const double2 f = make_double2((double)fs.x, (double)fs.y);
+                double2 v = cmul(make_double2(tk.pre_re, tk.pre_im), f);
+                if constexpr (NEAR != 0) {
+                    const double d = (tk.c - c_prev) * r2;
+                    if (since < kPhaseAnchor && fabs(d) <= 0x1p-7) {
+                        ph = cmul(ph, expi_small(d));
+                        since++;
+                    } else {
+                        if constexpr (NEAR == 1)
+                            sincos_reduced(tk.c * r2, &ph.y, &ph.x);
+                        else
+                            sincos(tk.c * r2, &ph.y, &ph.x);
+                        since = 0;
+                    }
+                    c_prev = tk.c;
+                    v = cmul(ph, v);
+                }
+                const double wv = wr * tk.w;
+                v.x *= wv;
+                v.y *= wv;
+                const double t = tk.s * r;
+                const double b = floor(t);
+                const int l0 = (int)b + 1;  // bin of node offset -(P/2 - 1): Q_l = (l - P/2) h
+                if (l0 < 0 || l0 + HK_GRID_P > nq) continue;  // the host sizes the grid so this never holds
+                window_move(w, l0, acc_re, acc_im);
+                window_add(w, t - b, v);
+            }
+            if (w.l0 >= 0) window_flush_all(w, acc_re, acc_im);
         }
     }
     __syncthreads();
-    const long long re_row = row0 + 2LL * o, im_row = re_row + 1;
-    for (int l = threadIdx.x; l < nq; l += blockDim.x) {
-        store_operand<TYPE>(out, re_row * ld + l, acc_re[l]);
-        store_operand<TYPE>(out, im_row * ld + l, acc_im[l]);
-    }
+    store_bins<TYPE>(acc_re, acc_im, nq, out, ld, row0 + 2LL * o);
 }
 
-template __global__ void hk_spread_kernel<HK_F64, false>(hk_planes, const double *, int, double, double, void *,
-                                                         long long, long long);
-template __global__ void hk_spread_kernel<HK_F32, false>(hk_planes, const double *, int, double, double, void *,
-                                                         long long, long long);
-template __global__ void hk_spread_kernel<HK_F64, true>(hk_planes, const double *, int, double, double, void *,
-                                                        long long, long long);
-template __global__ void hk_spread_kernel<HK_F32, true>(hk_planes, const double *, int, double, double, void *,
-                                                        long long, long long);
+#define HK_SPREAD_WINDOW(TYPE, S, NEAR)                                                                      \
+    template __global__ void hk_spread_window_kernel<TYPE, S, NEAR>(hk_planes, const double *, int, double, double, \
+                                                                    void *, long long, long long);
+HK_SPREAD_WINDOW(HK_F64, double2, 0)
+HK_SPREAD_WINDOW(HK_F64, double2, 1)
+HK_SPREAD_WINDOW(HK_F64, double2, 2)
+HK_SPREAD_WINDOW(HK_F64, float2, 0)
+HK_SPREAD_WINDOW(HK_F64, float2, 1)
+HK_SPREAD_WINDOW(HK_F64, float2, 2)
+HK_SPREAD_WINDOW(HK_F32, double2, 0)
+HK_SPREAD_WINDOW(HK_F32, double2, 1)
+HK_SPREAD_WINDOW(HK_F32, double2, 2)
+HK_SPREAD_WINDOW(HK_F32, float2, 0)
+HK_SPREAD_WINDOW(HK_F32, float2, 1)
+HK_SPREAD_WINDOW(HK_F32, float2, 2)
+#undef HK_SPREAD_WINDOW
 
 // J0(rho_m |Q_l|), Q_l = (l - P/2) h
 template <int TYPE>

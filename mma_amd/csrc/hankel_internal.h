@@ -27,9 +27,19 @@ template <typename S>
 __global__ void hk_direct_kernel(hk_planes p, int n_out, const double *weights, int per_plane, double2 *out,
                                  double inv_c, int uniform_r, double r0, double dr);
 
-template <int TYPE, bool TAB>
+template <int TYPE>
 __global__ void hk_spread_kernel(hk_planes p, const double *weights, int nq, double h, double inv_c, void *out,
                                  long long ld, long long row0);
+
+// range of sincos_reduced (hankel_contract.hip): |x| below it keeps n = rint(2x/pi) an exact int
+constexpr double kSincosLimit = 1e9;
+
+// hk_spread_window_kernel: doubles of per-plane factors in LDS after the bins (5 used, padded
+// to keep each plane's entry 16-byte aligned)
+constexpr int kSpreadTab = 6;
+template <int TYPE, typename S, int NEAR>
+__global__ void hk_spread_window_kernel(hk_planes p, const double *weights, int nq, double h, double inv_c, void *out,
+                                        long long ld, long long row0);
 
 template <int TYPE>
 __global__ void hk_bessel_kernel(int nrho, const double *rho, int nq, double h, void *out, long long ld);

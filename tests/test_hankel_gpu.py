@@ -208,6 +208,36 @@ def test_hankel_long_gridded_nonuniform_r_and_scalar_pressure():
         tgt = H.FSources_provider(zall, r, o, FSource=S)  # the provider is a single pass
 
 
+def test_spread_forms_agree_with_the_direct_contraction():
+    """Both forms of the spread against the direct FP64 contraction, on both outputs of a
+    weighted plane sum (all planes; one plane): the register-window form (a thread per radius
+    walking its planes), on uniform and non-uniform radii, and the per-item form the launcher
+    falls back to when the table of per-plane factors does not fit the LDS beside the bins
+    (3000 planes), each with and without the near-field phase."""
+    from mma_amd import hankel as H
+    rng = np.random.default_rng(5)
+    no, nr = 6, 70
+    o = OMEGA0 * np.linspace(21, 61, no)
+    rho = np.linspace(0, 7e-3, 64)
+    for nz, uniform in ((40, True), (40, False), (3000, True)):
+        z = np.linspace(0.0, 5e-3, nz)
+        r = np.linspace(0, 2e-4, nr) if uniform else np.sort(np.concatenate([[0.0], rng.uniform(0, 2e-4, nr - 1)]))
+        amp = np.exp(-(r / 6e-5) ** 2)[None, None, :]
+        src = torch.tensor(amp * np.exp(1j * rng.uniform(0, 6.28, (nz, no, nr))), device="cuda")
+        dz = np.diff(z)
+        w = np.zeros(nz)
+        w[:-1] += 0.5 * dz
+        w[1:] += 0.5 * dz
+        W = np.stack([w, np.eye(nz)[nz // 2]])
+        for near in (True, False):
+            ref = H.direct(src, z, o, r, rho, 1.0, weights=W, near_field=near).cpu().numpy()
+            got = H.gridded(src, z, o, r, rho, 1.0, W, near_field=near, kind="fp64-grid").cpu().numpy()
+            for q in range(2):
+                e = rel_err(got[q], ref[q])
+                print("spread nz %d uniform %s near %s output %d: %.2e" % (nz, uniform, near, q, e), flush=True)
+                assert e < 1e-9, (nz, uniform, near, q)
+
+
 def test_gemm_kernels_match_fp64():
     """hk_gemm (FP32 and bf16x3 matrix-core paths) against an FP64 product of the same FP32
     operands, with ragged m and n (tile edges)."""
