@@ -319,8 +319,8 @@ __device__ inline void window_flush_all(Window &w, double *acc_re, double *acc_i
 {
 #pragma unroll
     for (int m = 0; m < HK_GRID_P; m++) {
-        if (w.re[m] != 0.0) atomicAdd(&acc_re[w.l0 + m], w.re[m]);
-        if (w.im[m] != 0.0) atomicAdd(&acc_im[w.l0 + m], w.im[m]);
+        atomicAdd(&acc_re[w.l0 + m], w.re[m]);
+        atomicAdd(&acc_im[w.l0 + m], w.im[m]);
         w.re[m] = 0.0;
         w.im[m] = 0.0;
     }
@@ -349,30 +349,25 @@ __device__ inline void window_add(Window &w, double f, double2 v)
     }
 }
 
-// moves the stencil to start at bin l0 (0 <= l0 <= nq - HK_GRID_P)
+// moves the stencil to start at bin l0 (0 <= l0 <= nq - HK_GRID_P): a step of one bin (what
+// ascending planes give) flushes the bin left behind and shifts; any other move flushes all
 __device__ inline void window_move(Window &w, int l0, double *acc_re, double *acc_im)
 {
-    if (w.l0 < 0) {
-        w.l0 = l0;
-        return;
-    }
-    while (w.l0 != l0) {
-        if (l0 > w.l0 && l0 < w.l0 + HK_GRID_P) {
-            atomicAdd(&acc_re[w.l0], w.re[0]);
-            atomicAdd(&acc_im[w.l0], w.im[0]);
+    if (l0 == w.l0) return;
+    if (l0 == w.l0 + 1 && w.l0 >= 0) {
+        atomicAdd(&acc_re[w.l0], w.re[0]);
+        atomicAdd(&acc_im[w.l0], w.im[0]);
 #pragma unroll
-            for (int m = 0; m < HK_GRID_P - 1; m++) {
-                w.re[m] = w.re[m + 1];
-                w.im[m] = w.im[m + 1];
-            }
-            w.re[HK_GRID_P - 1] = 0.0;
-            w.im[HK_GRID_P - 1] = 0.0;
-            w.l0++;
-        } else {
-            window_flush_all(w, acc_re, acc_im);
-            w.l0 = l0;
+        for (int m = 0; m < HK_GRID_P - 1; m++) {
+            w.re[m] = w.re[m + 1];
+            w.im[m] = w.im[m + 1];
         }
+        w.re[HK_GRID_P - 1] = 0.0;
+        w.im[HK_GRID_P - 1] = 0.0;
+    } else if (w.l0 >= 0) {
+        window_flush_all(w, acc_re, acc_im);
     }
+    w.l0 = l0;
 }
 
 // A plane's factors for one frequency, read from the LDS table in one go.
