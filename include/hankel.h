@@ -7,7 +7,10 @@
  * HankelTransform :452-519, Hankel_long :522-687).  mma_amd/hankel.py mirrors those
  * signatures and calls the entry points below; they take DEVICE pointers (plain doubles,
  * complex values as interleaved (re, im) pairs), sizes and a hipStream_t, and return a
- * status (0 = OK) once the work queued on the stream has finished.
+ * status (0 = OK) once the work queued on the stream has finished -- except the three steps of
+ * the gridded contraction (hk_grid_spread, hk_grid_bessel, hk_gemm), which return once their
+ * kernel is queued (launch errors are still reported): their results are for later work on
+ * the same stream.
  *
  * Geometry of the far-field integral, for plane k at z_k, frequency omega_o and screen
  * radius rho_m (SI units, Hankel_transform.py:494-512):
@@ -78,6 +81,9 @@ typedef struct {
     int near_field;
     const double *pre;
     int pre_mode;
+    /* optional HOST copies of zgrid, ogrid and rgrid (NULL: the entry points that check the
+     * grids on the host copy them from the device, a synchronous transfer each) */
+    const double *host_zgrid, *host_ogrid, *host_rgrid;
 } hk_planes;
 
 /* Direct FP64 contraction: J0 evaluated in FP64 for every (plane, omega, rho, r) element

@@ -1,5 +1,6 @@
 // C ABI of libmmahankel.so (include/hankel.h): argument checks, launch shapes and the error
-// status; every entry point returns after the work it queued on `stream` has finished.
+// status; every entry point returns after the work it queued on `stream` has finished, except
+// the gridded contraction's steps (spread, Bessel matrix, GEMM), which return once queued.
 #include <algorithm>
 #include <cstdint>
 #include <cmath>
@@ -74,14 +75,22 @@ struct HostGrids {
     std::vector<double> r, z, o;
 };
 
+// one grid on the host: from the caller's host copy, else from the device
+int host_grid(std::vector<double> &v, const double *host, const double *dev, int n)
+{
+    v.resize(n);
+    if (host)
+        std::copy(host, host + n, v.begin());
+    else
+        HK_HIP(hipMemcpy(v.data(), dev, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return HK_OK;
+}
+
 int host_grids(const hk_planes *p, HostGrids &g)
 {
-    g.r.resize(p->nr);
-    g.z.resize(p->nz);
-    g.o.resize(p->no);
-    HK_HIP(hipMemcpy(g.r.data(), p->rgrid, sizeof(double) * p->nr, hipMemcpyDeviceToHost));
-    HK_HIP(hipMemcpy(g.z.data(), p->zgrid, sizeof(double) * p->nz, hipMemcpyDeviceToHost));
-    HK_HIP(hipMemcpy(g.o.data(), p->ogrid, sizeof(double) * p->no, hipMemcpyDeviceToHost));
+    if (host_grid(g.r, p->host_rgrid, p->rgrid, p->nr) != HK_OK || host_grid(g.z, p->host_zgrid, p->zgrid, p->nz) != HK_OK ||
+        host_grid(g.o, p->host_ogrid, p->ogrid, p->no) != HK_OK)
+        return g_status;
     return HK_OK;
 }
 
@@ -197,8 +206,8 @@ int hk_direct(const hk_planes *p, int n_out, const double *weights, int per_plan
     if (!out || (!per_plane && (n_out < 1 || n_out > 4 || !weights)))
         return fail(HK_ERR_ARGUMENT, "need out and 1..4 weighted outputs (or per_plane)");
     // a uniform radial grid lets the kernel rotate cos/sin of the Bessel argument
-    std::vector<double> r(p->nr);
-    HK_HIP(hipMemcpy(r.data(), p->rgrid, sizeof(double) * p->nr, hipMemcpyDeviceToHost));
+    std::vector<double> r;
+    if (host_grid(r, p->host_rgrid, p->rgrid, p->nr) != HK_OK) return g_status;
     double dr = 0.0;
     const bool uniform = uniform_radii(r, &dr);
     hipStream_t s = (hipStream_t)stream;
@@ -287,7 +296,6 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
         st = type == HK_F64 ? launch(hk_spread_kernel<HK_F64>) : launch(hk_spread_kernel<HK_F32>);
     if (st != HK_OK) return st;
     HK_HIP(hipGetLastError());
-    HK_HIP(hipStreamSynchronize(s));
     return HK_OK;
 }
 
@@ -305,7 +313,6 @@ int hk_grid_bessel(int nrho, const double *rho, int nq, double h, int type, void
     else
         return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");
     HK_HIP(hipGetLastError());
-    HK_HIP(hipStreamSynchronize(s));
     return HK_OK;
 }
 
@@ -327,7 +334,6 @@ int hk_gemm(int kind, int m, int n, int k, const float *a, long long lda, const 
     else
         return fail(HK_ERR_ARGUMENT, "unknown GEMM kind");
     HK_HIP(hipGetLastError());
-    HK_HIP(hipStreamSynchronize(s));
     return HK_OK;
 }
 

@@ -51,7 +51,8 @@ class hk_planes(Structure):
     _fields_ = [("nz", c_int), ("no", c_int), ("nr", c_int), ("nrho", c_int), ("src", c_void_p), ("src_c64", c_int),
                 ("plane_stride", c_longlong), ("zgrid", c_void_p), ("ogrid", c_void_p), ("rgrid", c_void_p),
                 ("tw", c_void_p), ("rho", c_void_p), ("distance", c_double), ("near_field", c_int),
-                ("pre", c_void_p), ("pre_mode", c_int)]
+                ("pre", c_void_p), ("pre_mode", c_int), ("host_zgrid", c_void_p), ("host_ogrid", c_void_p),
+                ("host_rgrid", c_void_p)]
 
 
 _lib = None
@@ -88,10 +89,33 @@ def _check(code):
     raise RuntimeError(msg)
 
 
+_DEV_SMALL = {}  # device copies of small host arrays (grids, weights), keyed by their bytes
+_DEV_SMALL_MAX = 64
+
+
 def _dev(a, dtype=torch.float64):
+    """A device copy of `a`.  Small host arrays (grids, plane weights) are cached by content,
+    so repeated calls with the same grids upload nothing; the cached tensors are only read."""
     if isinstance(a, torch.Tensor):
         return a.to(device="cuda", dtype=dtype).contiguous()
-    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+    h = np.ascontiguousarray(np.asarray(a, dtype=torch.empty((), dtype=dtype).numpy().dtype))
+    if h.size > 65536:
+        return torch.as_tensor(h, device="cuda")
+    key = (h.dtype.str, h.shape, h.tobytes(), torch.cuda.current_device())
+    t = _DEV_SMALL.get(key)
+    if t is None:
+        if len(_DEV_SMALL) >= _DEV_SMALL_MAX:
+            _DEV_SMALL.pop(next(iter(_DEV_SMALL)))
+        t = _DEV_SMALL[key] = torch.as_tensor(h, device="cuda")
+    return t
+
+
+def _host(a):
+    """A contiguous float64 host array for the ABI's optional host copies (None for tensors on
+    the device, which the library then copies back itself)."""
+    if isinstance(a, torch.Tensor):
+        return None if a.is_cuda else np.ascontiguousarray(a.numpy(), dtype=np.float64)
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
 
 
 def _ptr(t):
@@ -354,6 +378,10 @@ def _planes_struct(src, zgrid, ogrid, rgrid, rho, distance, near_field, pre, pre
     p.near_field = int(bool(near_field))
     p.pre = _ptr(pre)
     p.pre_mode = pre_mode
+    hz, ho, hr = _host(zgrid), _host(ogrid), _host(rgrid)
+    keep += [hz, ho, hr]
+    p.host_zgrid, p.host_ogrid, p.host_rgrid = (c_void_p(x.ctypes.data) if x is not None else c_void_p()
+                                                for x in (hz, ho, hr))
     return p
 
 
