@@ -101,16 +101,20 @@ int hk_direct(const hk_planes *p, int n_out, const double *weights, int per_plan
 #define HK_GRID_P 8
 int hk_grid_size(double q_max, double rho_max, double tol, int *nq, double *h);
 
-enum hk_operand { HK_F64 = 0, HK_F32 = 1 };
+/* Operand element types: double, float, or the value split into 2 (HK_BF16X2) or 3 (HK_BF16X3)
+ * bf16 parts v ~ p0 + p1 (+ p2), stored as 16-bit words with the parts of a row side by side
+ * (part q of element l of a row at row_start + q * nq + l; a row then holds parts * nq words
+ * and ld >= parts * nq): the operands of hk_gemm_split. */
+enum hk_operand { HK_F64 = 0, HK_F32 = 1, HK_BF16X2 = 2, HK_BF16X3 = 3 };
 
 /* C operand, "B^T" layout: for output q < n_out, row row0 + 2 (no q + o) holds the real
- * parts and the next row the imaginary parts of frequency o; nq columns, row stride ld >= nq;
- * type HK_F64 (double) or HK_F32 (float).  weights: n_out x nz plane weights (zero weights
- * skip the plane).  All outputs are spread in one launch. */
+ * parts and the next row the imaginary parts of frequency o; nq columns, row stride ld
+ * (elements of the type; >= nq, or >= parts * nq for the split types).  weights: n_out x nz
+ * plane weights (zero weights skip the plane).  All outputs are spread in one launch. */
 int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq, double h, int type, void *out,
                    long long ld, long long row0, void *stream);
 
-/* A operand: J0(rho_m |Q_l|), nrho rows x nq (row stride ld), HK_F64 or HK_F32. */
+/* A operand: J0(rho_m |Q_l|), nrho rows x nq (row stride ld), any hk_operand type. */
 int hk_grid_bessel(int nrho, const double *rho, int nq, double h, int type, void *out, long long ld, void *stream);
 
 enum hk_gemm_kind {
@@ -126,6 +130,14 @@ enum hk_gemm_kind {
  * multiples of 4. */
 int hk_gemm(int kind, int m, int n, int k, const float *a, long long lda, const float *b, long long ldb, double *out,
             long long ldo, void *stream);
+
+/* The same product from operands already split into `parts` (2 or 3) bf16 parts (HK_BF16X2,
+ * HK_BF16X3 layout: row r at a + r * lda, part q of it at + q * k; 16-bit words): parts 2 gives
+ * p0 q0 + p0 q1 + p1 q0, parts 3 the six products of HK_GEMM_BF16X6.  No splitting in the
+ * kernel, and the operand tiles are copied global -> LDS directly.  k a multiple of 64,
+ * lda, ldb >= parts * k and multiples of 8, 16-byte aligned operands. */
+int hk_gemm_split(int parts, int m, int n, int k, const void *a, long long lda, const void *b, long long ldb,
+                  double *out, long long ldo, void *stream);
 
 #ifdef __cplusplus
 }

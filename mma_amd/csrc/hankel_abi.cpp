@@ -113,6 +113,8 @@ int source_q_max(const hk_planes *p, const HostGrids &g, double *q_max)
     return HK_OK;
 }
 
+int operand_parts(int type) { return type == HK_BF16X2 ? 2 : type == HK_BF16X3 ? 3 : 1; }
+
 // r_j = r_0 + j dr to within a few ulps of max |r| (then angles advanced by rotation stay within
 // ~1e-13 rad of the exact ones)
 bool uniform_radii(const std::vector<double> &r, double *dr_out)
@@ -260,7 +262,8 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
     if (q_max / h > (double)(nq - HK_GRID_P - 1) * (1.0 + 1e-9))
         return fail(HK_ERR_ARGUMENT, "q grid too short for these sources (use hk_grid_size)");
     hipStream_t s = (hipStream_t)stream;
-    if (type != HK_F64 && type != HK_F32) return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");
+    if (type < HK_F64 || type > HK_BF16X3) return fail(HK_ERR_ARGUMENT, "unknown operand type");
+    if (ld < (long long)operand_parts(type) * nq) return fail(HK_ERR_ARGUMENT, "row stride below the operand's parts x nq");
     // bins, then (when they fit the LDS too) the per-plane factors of the register-window kernel
     const size_t lds_bins = (size_t)nq * sizeof(double2);
     const size_t lds_tab = lds_bins + (size_t)p->nz * kSpreadTab * sizeof(double);
@@ -287,13 +290,19 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
                    : near == 1 ? launch(hk_spread_window_kernel<T, S, 1>)
                                : launch(hk_spread_window_kernel<T, S, 2>);
         };
-        using F64 = std::integral_constant<int, HK_F64>;
-        using F32 = std::integral_constant<int, HK_F32>;
         struct D2 { using type = double2; };
         struct F2 { using type = float2; };
-        st = type == HK_F64 ? (c64 ? pick(F64{}, F2{}) : pick(F64{}, D2{})) : (c64 ? pick(F32{}, F2{}) : pick(F32{}, D2{}));
-    } else
-        st = type == HK_F64 ? launch(hk_spread_kernel<HK_F64>) : launch(hk_spread_kernel<HK_F32>);
+        auto by_src = [&](auto type_tag) { return c64 ? pick(type_tag, F2{}) : pick(type_tag, D2{}); };
+        st = type == HK_F64   ? by_src(std::integral_constant<int, HK_F64>{})
+             : type == HK_F32 ? by_src(std::integral_constant<int, HK_F32>{})
+             : type == HK_BF16X2 ? by_src(std::integral_constant<int, HK_BF16X2>{})
+                                 : by_src(std::integral_constant<int, HK_BF16X3>{});
+    } else {
+        st = type == HK_F64   ? launch(hk_spread_kernel<HK_F64>)
+             : type == HK_F32 ? launch(hk_spread_kernel<HK_F32>)
+             : type == HK_BF16X2 ? launch(hk_spread_kernel<HK_BF16X2>)
+                                 : launch(hk_spread_kernel<HK_BF16X3>);
+    }
     if (st != HK_OK) return st;
     HK_HIP(hipGetLastError());
     return HK_OK;
@@ -302,7 +311,9 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
 int hk_grid_bessel(int nrho, const double *rho, int nq, double h, int type, void *out, long long ld, void *stream)
 {
     clear_status();
-    if (nrho < 1 || !rho || nq < 1 || !out || ld < nq || !(h > 0)) return fail(HK_ERR_ARGUMENT, "bad Bessel arguments");
+    if (nrho < 1 || !rho || nq < 1 || !out || !(h > 0) || type < HK_F64 || type > HK_BF16X3 ||
+        ld < (long long)operand_parts(type) * nq)
+        return fail(HK_ERR_ARGUMENT, "bad Bessel arguments");
     hipStream_t s = (hipStream_t)stream;
     const long long n = (long long)nrho * nq;
     const dim3 grid((unsigned)((n + 255) / 256));
@@ -310,8 +321,10 @@ int hk_grid_bessel(int nrho, const double *rho, int nq, double h, int type, void
         hipLaunchKernelGGL(hk_bessel_kernel<HK_F64>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
     else if (type == HK_F32)
         hipLaunchKernelGGL(hk_bessel_kernel<HK_F32>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
+    else if (type == HK_BF16X2)
+        hipLaunchKernelGGL(hk_bessel_kernel<HK_BF16X2>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
     else
-        return fail(HK_ERR_ARGUMENT, "operand type must be HK_F64 or HK_F32");
+        hipLaunchKernelGGL(hk_bessel_kernel<HK_BF16X3>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
     HK_HIP(hipGetLastError());
     return HK_OK;
 }
@@ -333,6 +346,30 @@ int hk_gemm(int kind, int m, int n, int k, const float *a, long long lda, const 
         hipLaunchKernelGGL(hk_gemm_kernel<HK_GEMM_BF16X6>, grid, dim3(512), 0, s, m, n, k, a, lda, b, ldb, out, ldo);
     else
         return fail(HK_ERR_ARGUMENT, "unknown GEMM kind");
+    HK_HIP(hipGetLastError());
+    return HK_OK;
+}
+
+int hk_gemm_split(int parts, int m, int n, int k, const void *a, long long lda, const void *b, long long ldb,
+                  double *out, long long ldo, void *stream)
+{
+    clear_status();
+    if ((parts != 2 && parts != 3) || m < 1 || n < 1 || k < 64 || k % 64 != 0 || !a || !b || !out ||
+        lda < (long long)parts * k || ldb < (long long)parts * k || ldo < n || lda % 8 != 0 || ldb % 8 != 0 ||
+        ((uintptr_t)a & 15) != 0 || ((uintptr_t)b & 15) != 0)
+        return fail(HK_ERR_ARGUMENT, "split GEMM needs 2 or 3 parts, k a multiple of 64, strides >= parts x k and "
+                                     "multiples of 8, 16-byte aligned operands");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)((n + 127) / 128), (unsigned)((m + 127) / 128));
+    const size_t lds = hk_gemm_split_lds(parts);
+    auto launch = [&](auto kernel) -> int {
+        HK_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kernel, grid, dim3(512), lds, s, m, n, k, static_cast<const unsigned short *>(a), lda,
+                           static_cast<const unsigned short *>(b), ldb, out, ldo);
+        return HK_OK;
+    };
+    const int st = parts == 2 ? launch(hk_gemm_split_kernel<2>) : launch(hk_gemm_split_kernel<3>);
+    if (st != HK_OK) return st;
     HK_HIP(hipGetLastError());
     return HK_OK;
 }

@@ -203,13 +203,26 @@ struct Lagrange {
     }
 };
 
+// element l of an operand row starting at `row` (ld already applied): double, float, or the
+// value split into 2 or 3 bf16 parts v ~ p0 + p1 (+ p2), part q at row + q nq + l
 template <int TYPE>
-__device__ inline void store_operand(void *out, long long at, double v)
+__device__ inline void store_operand(void *out, long long row, int l, int nq, double v)
 {
-    if constexpr (TYPE == HK_F64)
-        reinterpret_cast<double *>(out)[at] = v;
-    else
-        reinterpret_cast<float *>(out)[at] = (float)v;
+    if constexpr (TYPE == HK_F64) {
+        reinterpret_cast<double *>(out)[row + l] = v;
+    } else if constexpr (TYPE == HK_F32) {
+        reinterpret_cast<float *>(out)[row + l] = (float)v;
+    } else {
+        constexpr int NP = TYPE == HK_BF16X2 ? 2 : 3;
+        unsigned short *o = reinterpret_cast<unsigned short *>(out) + row + l;
+        double r = v;
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            const __bf16 h = (__bf16)(float)r;
+            o[(long long)q * nq] = __builtin_bit_cast(unsigned short, h);
+            r -= (double)(float)h;
+        }
+    }
 }
 
 // Adds the Lagrange-weighted item v at grid position t to the bins (real and imaginary parts
@@ -232,8 +245,8 @@ __device__ inline void store_bins(const double *acc_re, const double *acc_im, in
                                   long long re_row)
 {
     for (int l = threadIdx.x; l < nq; l += blockDim.x) {
-        store_operand<TYPE>(out, re_row * ld + l, acc_re[l]);
-        store_operand<TYPE>(out, (re_row + 1) * ld + l, acc_im[l]);
+        store_operand<TYPE>(out, re_row * ld, l, nq, acc_re[l]);
+        store_operand<TYPE>(out, (re_row + 1) * ld, l, nq, acc_im[l]);
     }
 }
 
@@ -300,6 +313,10 @@ template __global__ void hk_spread_kernel<HK_F64>(hk_planes, const double *, int
                                                   long long);
 template __global__ void hk_spread_kernel<HK_F32>(hk_planes, const double *, int, double, double, void *, long long,
                                                   long long);
+template __global__ void hk_spread_kernel<HK_BF16X2>(hk_planes, const double *, int, double, double, void *, long long,
+                                                     long long);
+template __global__ void hk_spread_kernel<HK_BF16X3>(hk_planes, const double *, int, double, double, void *, long long,
+                                                     long long);
 
 // The same spread, accumulated in registers.  A thread owns one radius r_j (and a contiguous
 // range of planes) and walks its planes in order.  Its items' bins l0 = floor(K r_j / (D_k h)) + 1
@@ -521,6 +538,18 @@ HK_SPREAD_WINDOW(HK_F32, double2, 2)
 HK_SPREAD_WINDOW(HK_F32, float2, 0)
 HK_SPREAD_WINDOW(HK_F32, float2, 1)
 HK_SPREAD_WINDOW(HK_F32, float2, 2)
+HK_SPREAD_WINDOW(HK_BF16X2, double2, 0)
+HK_SPREAD_WINDOW(HK_BF16X2, double2, 1)
+HK_SPREAD_WINDOW(HK_BF16X2, double2, 2)
+HK_SPREAD_WINDOW(HK_BF16X2, float2, 0)
+HK_SPREAD_WINDOW(HK_BF16X2, float2, 1)
+HK_SPREAD_WINDOW(HK_BF16X2, float2, 2)
+HK_SPREAD_WINDOW(HK_BF16X3, double2, 0)
+HK_SPREAD_WINDOW(HK_BF16X3, double2, 1)
+HK_SPREAD_WINDOW(HK_BF16X3, double2, 2)
+HK_SPREAD_WINDOW(HK_BF16X3, float2, 0)
+HK_SPREAD_WINDOW(HK_BF16X3, float2, 1)
+HK_SPREAD_WINDOW(HK_BF16X3, float2, 2)
 #undef HK_SPREAD_WINDOW
 
 // J0(rho_m |Q_l|), Q_l = (l - P/2) h
@@ -531,8 +560,10 @@ __global__ void hk_bessel_kernel(int nrho, const double *rho, int nq, double h, 
     if (idx >= (long long)nrho * nq) return;
     const int m = (int)(idx / nq), l = (int)(idx % nq);
     const double q = (double)(l - HK_GRID_P / 2) * h;
-    store_operand<TYPE>(out, (long long)m * ld + l, hk::j0(rho[m] * q));
+    store_operand<TYPE>(out, (long long)m * ld, l, nq, hk::j0(rho[m] * q));
 }
 
 template __global__ void hk_bessel_kernel<HK_F64>(int, const double *, int, double, void *, long long);
 template __global__ void hk_bessel_kernel<HK_F32>(int, const double *, int, double, void *, long long);
+template __global__ void hk_bessel_kernel<HK_BF16X2>(int, const double *, int, double, void *, long long);
+template __global__ void hk_bessel_kernel<HK_BF16X3>(int, const double *, int, double, void *, long long);

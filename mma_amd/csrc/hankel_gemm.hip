@@ -20,10 +20,11 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 namespace {
-constexpr int BM = 128, BN = 128, BK = 32;
+constexpr int BM = 128, BN = 128, BK = 64;
 constexpr int kWaves = 8, kThreads = 64 * kWaves;  // 4 x 2 waves of 32 x 64
 constexpr int kLoads = BM * BK / 4 / kThreads;      // float4 loads per thread per operand and stage
-constexpr int kFold = 2;  // K stages summed in FP32 before the FP64 accumulators
+constexpr int kVecRow = BK / 4;                     // float4 per tile row
+constexpr int kFold = 1;  // K stages summed in FP32 before the FP64 accumulators
 constexpr int kPadF = 1;  // float row padding: conflict-free column reads
 constexpr int kPadH = 8;  // bf16 row padding: keeps 16-byte alignment of 8-element reads
 
@@ -61,7 +62,7 @@ hk_gemm_kernel(int M, int N, int K, const float *A, long long lda, const float *
     auto load_stage = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < kLoads; i++) {
-            const int lin = i * kThreads + tid, row = lin >> 3, c4 = lin & 7;
+            const int lin = i * kThreads + tid, row = lin / kVecRow, c4 = lin % kVecRow;
             const long long gm = m0 + row, gn = n0 + row;
             ra[i] = gm < M ? *reinterpret_cast<const float4 *>(A + gm * lda + k0 + 4 * c4) : zero4;
             rb[i] = gn < N ? *reinterpret_cast<const float4 *>(B + gn * ldb + k0 + 4 * c4) : zero4;
@@ -80,7 +81,7 @@ hk_gemm_kernel(int M, int N, int K, const float *A, long long lda, const float *
         __syncthreads();  // the previous tile has been consumed
 #pragma unroll
         for (int i = 0; i < kLoads; i++) {
-            const int lin = i * kThreads + tid, row = lin >> 3, c = 4 * (lin & 7);
+            const int lin = i * kThreads + tid, row = lin / kVecRow, c = 4 * (lin % kVecRow);
             float xa[4] = {ra[i].x, ra[i].y, ra[i].z, ra[i].w};
             float xb[4] = {rb[i].x, rb[i].y, rb[i].z, rb[i].w};
             if constexpr (SPLIT) {
@@ -170,3 +171,143 @@ template __global__ void hk_gemm_kernel<HK_GEMM_BF16X3>(int, int, int, const flo
                                                         long long, double *, long long);
 template __global__ void hk_gemm_kernel<HK_GEMM_BF16X6>(int, int, int, const float *, long long, const float *,
                                                         long long, double *, long long);
+
+// ---- pre-split operands ---------------------------------------------------------------------
+// A and B already split into NP bf16 parts by their producers, the parts of a row side by side
+// (part q of row r at r * ld + q * K).  K is staged through LDS SBK at a time in two buffers:
+// while the MFMAs read one stage, the next is copied from global memory straight into the
+// other buffer (global_load_lds_dwordx4, no registers), so a stage costs one round trip of
+// latency hidden behind the previous stage's MFMAs.  Tile rows (A rows, then B rows) are
+// SBK bf16 long; the 16-byte chunk c of row r sits at slot c ^ ((r / RPB) & (CH - 1)), RPB the
+// rows in 256 bytes, so the 32 rows an MFMA operand read touches spread over all banks.
+// one 64-lane copy of 16 bytes per lane from global memory to LDS (lane i writes lds + 16 i);
+// a device function of its own: the builtin exists only in the device compilation
+__device__ inline void copy_to_lds(const unsigned short *g, unsigned short *lds)
+{
+    __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
+}
+
+template <int NP>
+struct SplitShape {
+    static constexpr int SBK = NP == 2 ? 64 : 32;      // K per stage
+    static constexpr int CH = SBK / 8;                  // 16-byte chunks per tile row
+    static constexpr int RPB = 256 / (2 * SBK);         // tile rows per 256 bytes
+    static constexpr int ROWS = BM + BN;                // A rows, then B rows
+    static constexpr int TILE = NP * ROWS * SBK;        // bf16 per stage buffer
+    static constexpr int RPI = 64 / CH;                 // tile rows per 64-lane copy
+    static constexpr int PER_WAVE = NP * ROWS / RPI / kWaves;  // copies per wave per stage
+    static constexpr int FOLD = 64 / SBK;               // stages per FP64 fold (64 K terms)
+};
+
+template <int NP>
+__global__ void __launch_bounds__(kThreads, 1)
+hk_gemm_split_kernel(int M, int N, int K, const unsigned short *A, long long lda, const unsigned short *B,
+                     long long ldb, double *out, long long ldo)
+{
+    using S = SplitShape<NP>;
+    extern __shared__ __attribute__((aligned(16))) unsigned short lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;  // rows 32 wm .. +32, columns 64 wn .. +64
+    const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
+    const int r32 = lane & 31, kh = lane >> 5;
+    const int rr = lane / S::CH, cs = lane % S::CH;  // this lane's row and slot in a copy
+
+    auto issue = [&](int k0, int buf) {
+#pragma unroll
+        for (int u = 0; u < S::PER_WAVE; u++) {
+            const int ins = wave * S::PER_WAVE + u;                 // uniform
+            const int q = ins / (S::ROWS / S::RPI);
+            const int row0 = (ins % (S::ROWS / S::RPI)) * S::RPI;  // first tile row of the copy
+            const int row = row0 + rr;
+            const int c = cs ^ ((row / S::RPB) & (S::CH - 1));      // the chunk that lands in slot cs
+            const unsigned short *g;
+            if (row0 < BM)  // uniform: a copy is all A rows or all B rows
+                g = A + min(m0 + row, (long long)M - 1) * lda + (long long)q * K + k0 + 8 * c;
+            else
+                g = B + min(n0 + row - BM, (long long)N - 1) * ldb + (long long)q * K + k0 + 8 * c;
+            copy_to_lds(g, lds + buf * S::TILE + (q * S::ROWS + row0) * S::SBK);
+        }
+    };
+    auto operand = [&](int buf, int q, int row, int c) {
+        const int slot = c ^ ((row / S::RPB) & (S::CH - 1));
+        return *reinterpret_cast<const bf16x8 *>(lds + buf * S::TILE + (q * S::ROWS + row) * S::SBK + 8 * slot);
+    };
+
+    double accd[2][16];
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) accd[tj][v] = 0.0;
+    f32x16 acc[2];
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[tj][v] = 0.0f;
+
+    const int stages = K / S::SBK;
+    issue(0, 0);
+    for (int s = 0; s < stages; s++) {
+        const int buf = s & 1;
+        if (s + 1 < stages) {
+            issue((s + 1) * S::SBK, buf ^ 1);
+            // this wave's copies of stage s are done when only stage s + 1's remain
+            if constexpr (S::PER_WAVE == 8)
+                asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+            else
+                asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+        } else {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+        __builtin_amdgcn_s_barrier();  // every wave's copies of stage s are in LDS
+#pragma unroll
+        for (int kk = 0; kk < S::SBK; kk += 16) {
+            const int c = kk / 8 + kh;
+            bf16x8 ap[NP], bp[NP][2];
+#pragma unroll
+            for (int q = 0; q < NP; q++) {
+                ap[q] = operand(buf, q, wm * 32 + r32, c);
+#pragma unroll
+                for (int t = 0; t < 2; t++) bp[q][t] = operand(buf, q, BM + wn * 64 + t * 32 + r32, c);
+            }
+#pragma unroll
+            for (int tj = 0; tj < 2; tj++) {
+                f32x16 v = acc[tj];
+                if constexpr (NP == 3) {  // smallest terms first
+                    v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[2], bp[0][tj], v, 0, 0, 0);
+                    v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1], bp[1][tj], v, 0, 0, 0);
+                    v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[2][tj], v, 0, 0, 0);
+                }
+                v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1], bp[0][tj], v, 0, 0, 0);
+                v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[1][tj], v, 0, 0, 0);
+                v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[0][tj], v, 0, 0, 0);
+                acc[tj] = v;
+            }
+        }
+        if ((s + 1) % S::FOLD == 0 || s + 1 == stages) {
+#pragma unroll
+            for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    accd[tj][v] += (double)acc[tj][v];
+                    acc[tj][v] = 0.0f;
+                }
+        }
+        __builtin_amdgcn_s_barrier();  // buffer `buf` is free for stage s + 2
+    }
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const long long row = m0 + wm * 32 + 8 * (v >> 2) + 4 * kh + (v & 3);
+            const long long col = n0 + wn * 64 + tj * 32 + r32;
+            if (row < M && col < N) out[row * ldo + col] = accd[tj][v];
+        }
+}
+
+template __global__ void hk_gemm_split_kernel<2>(int, int, int, const unsigned short *, long long,
+                                                 const unsigned short *, long long, double *, long long);
+template __global__ void hk_gemm_split_kernel<3>(int, int, int, const unsigned short *, long long,
+                                                 const unsigned short *, long long, double *, long long);
+
+size_t hk_gemm_split_lds(int np) { return np == 2 ? sizeof(unsigned short) * 2 * SplitShape<2>::TILE
+                                                  : sizeof(unsigned short) * 2 * SplitShape<3>::TILE; }

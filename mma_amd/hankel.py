@@ -33,11 +33,10 @@ TABLES = os.path.join(HERE, "data", "xuv_tables.npz")
 C_LIGHT = 299792458.0  # shared_python/units.py:3
 
 HK_OK, HK_ERR_ARGUMENT, HK_ERR_UNSUPPORTED, HK_ERR_DEVICE, HK_ERR_TABLE = 0, 3, 4, 5, 7
-HK_F64, HK_F32 = 0, 1
+HK_F64, HK_F32, HK_BF16X2, HK_BF16X3 = 0, 1, 2, 3
 HK_GEMM_F32, HK_GEMM_BF16X3, HK_GEMM_BF16X6 = 1, 2, 3
 GRID_TOL = 1e-10  # bound on the J0 interpolation error of the gridded contraction
 CONTRACTIONS = ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma")
-_GEMM_KIND = {"fp32-mfma": HK_GEMM_F32, "bf16x3-mfma": HK_GEMM_BF16X3, "bf16x6-mfma": HK_GEMM_BF16X6}
 
 
 class hk_medium(Structure):
@@ -74,6 +73,8 @@ def _L():
         L.hk_grid_bessel.argtypes = [c_int, c_void_p, c_int, c_double, c_int, c_void_p, c_longlong, c_void_p]
         L.hk_gemm.argtypes = [c_int, c_int, c_int, c_int, c_void_p, c_longlong, c_void_p, c_longlong, c_void_p,
                               c_longlong, c_void_p]
+        L.hk_gemm_split.argtypes = [c_int, c_int, c_int, c_int, c_void_p, c_longlong, c_void_p, c_longlong, c_void_p,
+                                    c_longlong, c_void_p]
         _lib = L
     return _lib
 
@@ -425,20 +426,26 @@ def gridded(src, zgrid, ogrid, rgrid, rho, distance, weights, near_field=True, p
     nq, h = grid_size_of(zgrid, ogrid, rgrid, rho, distance, tol)
     W = np.atleast_2d(np.asarray(weights, dtype=np.float64))
     n_out = W.shape[0]
-    otype = HK_F64 if kind == "fp64-grid" else HK_F32
-    dt = torch.float64 if otype == HK_F64 else torch.float32
-    Bm = torch.empty((n_out * 2 * p.no, nq), dtype=dt, device="cuda")
+    # operands: FP64 (library GEMM), FP32 (FP32 MFMA) or pre-split bf16 parts (bf16 MFMA)
+    parts = {"bf16x3-mfma": 2, "bf16x6-mfma": 3}.get(kind, 1)
+    otype = {"fp64-grid": HK_F64, "fp32-mfma": HK_F32, "bf16x3-mfma": HK_BF16X2, "bf16x6-mfma": HK_BF16X3}[kind]
+    dt = {HK_F64: torch.float64, HK_F32: torch.float32}.get(otype, torch.int16)
+    Bm = torch.empty((n_out * 2 * p.no, parts * nq), dtype=dt, device="cuda")
     wq = _dev(np.ascontiguousarray(W))
-    _check(L.hk_grid_spread(byref(p), n_out, _ptr(wq), nq, h, otype, _ptr(Bm), nq, 0, _stream()))
+    _check(L.hk_grid_spread(byref(p), n_out, _ptr(wq), nq, h, otype, _ptr(Bm), parts * nq, 0, _stream()))
     rh = _dev(rho)
-    Am = torch.empty((p.nrho, nq), dtype=dt, device="cuda")
-    _check(L.hk_grid_bessel(p.nrho, _ptr(rh), nq, h, otype, _ptr(Am), nq, _stream()))
+    Am = torch.empty((p.nrho, parts * nq), dtype=dt, device="cuda")
+    _check(L.hk_grid_bessel(p.nrho, _ptr(rh), nq, h, otype, _ptr(Am), parts * nq, _stream()))
     if kind == "fp64-grid":
         out = Am @ Bm.T  # plain FP64 library GEMM
     else:
         out = torch.empty((p.nrho, Bm.shape[0]), dtype=torch.float64, device="cuda")
-        k = _GEMM_KIND[kind]
-        _check(L.hk_gemm(k, p.nrho, Bm.shape[0], nq, _ptr(Am), nq, _ptr(Bm), nq, _ptr(out), Bm.shape[0], _stream()))
+        if parts > 1:
+            _check(L.hk_gemm_split(parts, p.nrho, Bm.shape[0], nq, _ptr(Am), parts * nq, _ptr(Bm), parts * nq,
+                                   _ptr(out), Bm.shape[0], _stream()))
+        else:
+            _check(L.hk_gemm(HK_GEMM_F32, p.nrho, Bm.shape[0], nq, _ptr(Am), nq, _ptr(Bm), nq, _ptr(out), Bm.shape[0],
+                             _stream()))
     out = out.view(p.nrho, n_out, p.no, 2).permute(1, 0, 2, 3).contiguous()
     return torch.view_as_complex(out)
 

@@ -255,6 +255,41 @@ def test_gemm_kernels_match_fp64():
             assert e < tol
 
 
+def _bf16_parts(x, n):
+    """x (float64) as n bf16 parts, each the round-to-nearest-even bf16 of the float32 of what
+    the earlier parts leave (the kernels' split), as 16-bit words."""
+    parts, r = [], np.asarray(x, dtype=np.float64).copy()
+    for _ in range(n):
+        b = r.astype(np.float32).view(np.uint32).astype(np.uint64)
+        w = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+        parts.append(w)
+        r = r - (w.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    return parts
+
+
+def test_split_gemm_matches_fp64():
+    """hk_gemm_split on operands split on the host: the products of the kept part pairs (two
+    parts: p0 q0 + p0 q1 + p1 q0; three: the six of order <= 2^-16) against their FP64 sum,
+    with ragged m and n (tile edges) and the far-field workload's K = 5120."""
+    from mma_amd import hankel as H
+    rng = np.random.default_rng(11)
+    pairs = {2: [(0, 0), (0, 1), (1, 0)], 3: [(0, 0), (0, 1), (1, 0), (0, 2), (1, 1), (2, 0)]}
+    for m, n, k in ((200, 330, 256), (1024, 600, 5120)):
+        A, B = rng.standard_normal((m, k)), rng.standard_normal((n, k))
+        for parts in (2, 3):
+            ap, bp = _bf16_parts(A, parts), _bf16_parts(B, parts)
+            val = lambda w: (w.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+            ref = sum(val(ap[i]) @ val(bp[j]).T for i, j in pairs[parts])
+            Ad = torch.tensor(np.concatenate(ap, axis=1).view(np.int16), device="cuda")
+            Bd = torch.tensor(np.concatenate(bp, axis=1).view(np.int16), device="cuda")
+            out = torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda")
+            H._check(H._L().hk_gemm_split(parts, m, n, k, H._ptr(Ad), parts * k, H._ptr(Bd), parts * k, H._ptr(out), n,
+                                          H._stream()))
+            e = rel_err(out.cpu().numpy(), ref)
+            print("split gemm %dx%dx%d parts %d: %.2e" % (m, n, k, parts, e), flush=True)
+            assert e < 1e-6
+
+
 def test_full_size_far_field_error_bound():
     """The far-field workload at full size (256 planes x 2048 frequencies x 512 radii -> 1024
     screen radii at 1 m, Ar cell with a z-modulated pressure, complex128 and complex64 sources)
