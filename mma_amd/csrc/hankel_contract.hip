@@ -4,6 +4,8 @@
 //   * hk_spread_kernel + hk_bessel_kernel: the gridded form, C_l(omega) spread from the
 //     sources by Lagrange interpolation in q = K r / D and the matrix J0(rho_m Q_l) whose
 //     product (hankel_gemm.hip) gives the same weighted sum over planes.
+#include <type_traits>
+
 #include "hankel_internal.h"
 #include "hankel_j0.h"
 
@@ -327,60 +329,82 @@ template __global__ void hk_spread_kernel<HK_BF16X3>(hk_planes, const double *, 
 // item.  Per-plane factors (K / D / h, -K / (2 D), weight, pre-factor) sit in LDS after the
 // bins (kSpreadTab doubles per plane).  Any plane order is handled (a backward or far jump
 // flushes the whole stencil).
+// Arithmetic of the register stencil: double, or float where the operand keeps fewer bits
+// than float does (two bf16 parts hold 16).
+template <typename R>
+using cplx = typename std::conditional<std::is_same<R, double>::value, double2, float2>::type;
+template <typename R>
+__device__ inline cplx<R> mkc(R re, R im)
+{
+    cplx<R> v;
+    v.x = re;
+    v.y = im;
+    return v;
+}
+template <typename R>
+__device__ inline cplx<R> cmulr(cplx<R> a, cplx<R> b)
+{
+    return mkc<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <typename R>
 struct Window {
-    double re[HK_GRID_P], im[HK_GRID_P];
+    R re[HK_GRID_P], im[HK_GRID_P];
     int l0;  // bin of re[0]; below 0: empty
 };
 
-__device__ inline void window_flush_all(Window &w, double *acc_re, double *acc_im)
+template <typename R>
+__device__ inline void window_flush_all(Window<R> &w, double *acc_re, double *acc_im)
 {
 #pragma unroll
     for (int m = 0; m < HK_GRID_P; m++) {
-        atomicAdd(&acc_re[w.l0 + m], w.re[m]);
-        atomicAdd(&acc_im[w.l0 + m], w.im[m]);
-        w.re[m] = 0.0;
-        w.im[m] = 0.0;
+        atomicAdd(&acc_re[w.l0 + m], (double)w.re[m]);
+        atomicAdd(&acc_im[w.l0 + m], (double)w.im[m]);
+        w.re[m] = R(0);
+        w.im[m] = R(0);
     }
 }
 
 // adds v times the Lagrange basis at f (the same weights as Lagrange(f)) to the stencil, each
 // weight formed as it is used: the suffix products are kept, the prefix product runs
-__device__ inline void window_add(Window &w, double f, double2 v)
+template <typename R>
+__device__ inline void window_add(Window<R> &w, R f, cplx<R> v)
 {
     constexpr int P = HK_GRID_P;
-    double suf[P];
-    suf[P - 1] = 1.0;
+    R suf[P];
+    suf[P - 1] = R(1);
 #pragma unroll
-    for (int i = P - 2; i >= 0; i--) suf[i] = suf[i + 1] * (f - (double)(i + 1 - (P / 2 - 1)));
-    double pre = 1.0;
+    for (int i = P - 2; i >= 0; i--) suf[i] = suf[i + 1] * (f - R(i + 1 - (P / 2 - 1)));
+    R pre = R(1);
 #pragma unroll
     for (int i = 0; i < P; i++) {
         double den = 1.0;
         for (int q = 2; q <= i; q++) den *= (double)q;
         for (int q = 2; q <= P - 1 - i; q++) den *= (double)q;
-        const double inv = ((P - 1 - i) & 1 ? -1.0 : 1.0) / den;  // folded at compile time
-        const double L = (pre * suf[i]) * inv;
+        const R inv = R(((P - 1 - i) & 1 ? -1.0 : 1.0) / den);  // folded at compile time
+        const R L = (pre * suf[i]) * inv;
         w.re[i] = fma(L, v.x, w.re[i]);
         w.im[i] = fma(L, v.y, w.im[i]);
-        pre *= f - (double)(i - (P / 2 - 1));
+        pre *= f - R(i - (P / 2 - 1));
     }
 }
 
 // moves the stencil to start at bin l0 (0 <= l0 <= nq - HK_GRID_P): a step of one bin (what
 // ascending planes give) flushes the bin left behind and shifts; any other move flushes all
-__device__ inline void window_move(Window &w, int l0, double *acc_re, double *acc_im)
+template <typename R>
+__device__ inline void window_move(Window<R> &w, int l0, double *acc_re, double *acc_im)
 {
     if (l0 == w.l0) return;
     if (l0 == w.l0 + 1 && w.l0 >= 0) {
-        atomicAdd(&acc_re[w.l0], w.re[0]);
-        atomicAdd(&acc_im[w.l0], w.im[0]);
+        atomicAdd(&acc_re[w.l0], (double)w.re[0]);
+        atomicAdd(&acc_im[w.l0], (double)w.im[0]);
 #pragma unroll
         for (int m = 0; m < HK_GRID_P - 1; m++) {
             w.re[m] = w.re[m + 1];
             w.im[m] = w.im[m + 1];
         }
-        w.re[HK_GRID_P - 1] = 0.0;
-        w.im[HK_GRID_P - 1] = 0.0;
+        w.re[HK_GRID_P - 1] = R(0);
+        w.im[HK_GRID_P - 1] = R(0);
     } else if (w.l0 >= 0) {
         window_flush_all(w, acc_re, acc_im);
     }
@@ -400,17 +424,18 @@ __device__ inline PlaneTab plane_tab(const double *tab, int k)
 
 // e^{i x} for |x| <= 2^-7: Taylor series through x^7 (sin) and x^8 (cos), first omitted terms
 // below 1e-24
-__device__ inline double2 expi_small(double x)
+template <typename R>
+__device__ inline cplx<R> expi_small(R x)
 {
-    const double z = x * x;
-    double ps = -1.0 / 5040.0;
-    ps = fma(ps, z, 1.0 / 120.0);
-    ps = fma(ps, z, -1.0 / 6.0);
-    double pc = 1.0 / 40320.0;
-    pc = fma(pc, z, -1.0 / 720.0);
-    pc = fma(pc, z, 1.0 / 24.0);
-    pc = fma(pc, z, -0.5);
-    return make_double2(fma(z, pc, 1.0), fma(x * z, ps, x));
+    const R z = x * x;
+    R ps = R(-1.0 / 5040.0);
+    ps = fma(ps, z, R(1.0 / 120.0));
+    ps = fma(ps, z, R(-1.0 / 6.0));
+    R pc = R(1.0 / 40320.0);
+    pc = fma(pc, z, R(-1.0 / 720.0));
+    pc = fma(pc, z, R(1.0 / 24.0));
+    pc = fma(pc, z, R(-0.5));
+    return mkc<R>(fma(z, pc, R(1)), fma(x * z, ps, x));
 }
 
 constexpr int kPhaseAnchor = 32;  // planes between exact near-field phases
@@ -418,11 +443,14 @@ constexpr int kPhaseAnchor = 32;  // planes between exact near-field phases
 // S: the source element type (double2 or float2), kept as loaded while it is in flight.
 // NEAR: 0 no near-field phase, 1 its sincos by sincos_reduced (|phase| < kSincosLimit / 2,
 // checked by the host), 2 by the general sincos.
+// R: float for the two-bf16-part operand (16 bits kept; the item's grid position t and the
+// phase steps' angle stay in double), double otherwise.
 template <int TYPE, typename S, int NEAR>
 __global__ void __launch_bounds__(kSpreadThreads)
 hk_spread_window_kernel(hk_planes p, const double *weights, int nq, double h, double inv_c, void *out, long long ld,
                         long long row0)
 {
+    using R = typename std::conditional<TYPE == HK_BF16X2, float, double>::type;
     extern __shared__ double bins[];
     __shared__ int k_range[2];
     double *acc_re = bins, *acc_im = bins + nq, *tab = bins + 2 * nq;
@@ -468,17 +496,17 @@ hk_spread_window_kernel(hk_planes p, const double *weights, int nq, double h, do
             const double r = p.rgrid[j];
             const double wr = p.tw[j] * r;
             const S *src = reinterpret_cast<const S *>(p.src) + (long long)ka * p.plane_stride + (long long)o * p.nr + j;
-            Window w;
+            Window<R> w;
             w.l0 = -1;
 #pragma unroll
             for (int m = 0; m < HK_GRID_P; m++) {
-                w.re[m] = 0.0;
-                w.im[m] = 0.0;
+                w.re[m] = R(0);
+                w.im[m] = R(0);
             }
             // the near-field phase e^{i c_k r^2} advances from plane to plane by the small angle
             // (c_k - c_prev) r^2 (consecutive planes: ~K r^2 dz / (2 D^2)); an exact sincos where
             // that angle is not small, and every kPhaseAnchor planes
-            double2 ph = make_double2(1.0, 0.0);
+            cplx<R> ph = mkc<R>(R(1), R(0));
             double c_prev = 0.0;
             int since = kPhaseAnchor;
             const double r2 = r * r;
@@ -489,32 +517,34 @@ hk_spread_window_kernel(hk_planes p, const double *weights, int nq, double h, do
                 if (k + 1 < kb) f1 = src[0];  // the next plane's source in flight
                 const PlaneTab tk = plane_tab(tab, k);
                 if (tk.w == 0.0) continue;
-                const double2 f = make_double2((double)fs.x, (double)fs.y);
-                double2 v = cmul(make_double2(tk.pre_re, tk.pre_im), f);
+                cplx<R> v = cmulr<R>(mkc<R>(R(tk.pre_re), R(tk.pre_im)), mkc<R>(R(fs.x), R(fs.y)));
                 if constexpr (NEAR != 0) {
-                    const double d = (tk.c - c_prev) * r2;
+                    const double d = (tk.c - c_prev) * r2;  // in double: c_k - c_prev cancels
                     if (since < kPhaseAnchor && fabs(d) <= 0x1p-7) {
-                        ph = cmul(ph, expi_small(d));
+                        ph = cmulr<R>(ph, expi_small<R>(R(d)));
                         since++;
                     } else {
-                        if constexpr (NEAR == 1)
+                        if constexpr (std::is_same<R, float>::value) {
+                            sincosf((float)(tk.c * r2), &ph.y, &ph.x);
+                        } else if constexpr (NEAR == 1) {
                             sincos_reduced(tk.c * r2, &ph.y, &ph.x);
-                        else
+                        } else {
                             sincos(tk.c * r2, &ph.y, &ph.x);
+                        }
                         since = 0;
                     }
                     c_prev = tk.c;
-                    v = cmul(ph, v);
+                    v = cmulr<R>(ph, v);
                 }
-                const double wv = wr * tk.w;
+                const R wv = R(wr * tk.w);
                 v.x *= wv;
                 v.y *= wv;
-                const double t = tk.s * r;
+                const double t = tk.s * r;  // in double: its fraction places the item on the grid
                 const double b = floor(t);
                 const int l0 = (int)b + 1;  // bin of node offset -(P/2 - 1): Q_l = (l - P/2) h
                 if (l0 < 0 || l0 + HK_GRID_P > nq) continue;  // the host sizes the grid so this never holds
                 window_move(w, l0, acc_re, acc_im);
-                window_add(w, t - b, v);
+                window_add(w, R(t - b), v);
             }
             if (w.l0 >= 0) window_flush_all(w, acc_re, acc_im);
         }
