@@ -163,7 +163,7 @@ def _known_trapezoids():
     return known
 
 
-_TRAPEZOID_VERDICT = {}
+_TRAPEZOID_PROBED = {}
 
 
 def _is_trapezoid(integrator):
@@ -174,7 +174,7 @@ def _is_trapezoid(integrator):
     if any(integrator is k for k in _known_trapezoids()):
         return True
     key = id(integrator)
-    if key not in _TRAPEZOID_VERDICT:
+    if key not in _TRAPEZOID_PROBED:
         x = np.array([0.0, 0.3, 0.35, 1.1, 1.7, 2.0])
         y = np.array([1.0, -2.0 + 1j, 0.5 - 3j, 4.0, -1.5j, 2.5 + 0.25j])
         want = np.sum(0.5 * (x[1:] - x[:-1]) * (y[1:] + y[:-1]))
@@ -183,8 +183,8 @@ def _is_trapezoid(integrator):
             ok = abs(got - want) <= 1e-12 * abs(want)
         except Exception:
             ok = False
-        _TRAPEZOID_VERDICT[key] = (integrator, ok)  # the callable is held so its id stays unique
-    return _TRAPEZOID_VERDICT[key][1]
+        _TRAPEZOID_PROBED[key] = (integrator, ok)  # the callable is held so its id stays unique
+    return _TRAPEZOID_PROBED[key][1]
 
 
 def _trapezoid_weights(x):
