@@ -332,7 +332,7 @@ def hankel_measure(cpu=True):
         return H.gridded(src, zg, og, rg, rho, distance, Wt, pre=tab.pre, kind=kind)
 
     out, res = {}, {}
-    for kind in ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma"):
+    for kind in ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma", "fp16x3-mfma"):
         run(kind)  # warm-up (code objects, allocations)
         torch.cuda.synchronize()
         reps = 1 if kind == "fp64" else 5
@@ -347,7 +347,7 @@ def hankel_measure(cpu=True):
     errors = {}
     # each output against its own scale: the integrated field (a sum over planes weighted by the
     # plane spacing) is ~1/Nz of a plane's magnitude, so a max over all three would hide its error
-    for kind in ("fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma"):
+    for kind in ("fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma", "fp16x3-mfma"):
         errors[kind] = {}
         for i, name in enumerate(("FF_integrated", "entry_plane_transform", "exit_plane_transform")):
             d = res[kind][i] - ref[i]

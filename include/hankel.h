@@ -105,7 +105,11 @@ int hk_grid_size(double q_max, double rho_max, double tol, int *nq, double *h);
  * bf16 parts v ~ p0 + p1 (+ p2), stored as 16-bit words with the parts of a row side by side
  * (part q of element l of a row at row_start + q * nq + l; a row then holds parts * nq words
  * and ld >= parts * nq): the operands of hk_gemm_split. */
-enum hk_operand { HK_F64 = 0, HK_F32 = 1, HK_BF16X2 = 2, HK_BF16X3 = 3 };
+enum hk_operand { HK_F64 = 0, HK_F32 = 1, HK_BF16X2 = 2, HK_BF16X3 = 3, HK_F16X2 = 4 };
+/* HK_F16X2: each row scaled by a power of two s (largest element in [2^13, 2^14)) and split into
+ * two fp16 parts, v s ~ h0 + 2^-11 h1 (part q at row_start + q * nq + l), the double s at
+ * row_start + 2 * nq (16-bit words; ld >= 2 * nq + 4, a multiple of 8): the operands of
+ * hk_gemm_f16x2. */
 
 /* C operand, "B^T" layout: for output q < n_out, row row0 + 2 (no q + o) holds the real
  * parts and the next row the imaginary parts of frequency o; nq columns, row stride ld
@@ -138,6 +142,12 @@ int hk_gemm(int kind, int m, int n, int k, const float *a, long long lda, const 
  * lda, ldb >= parts * k and multiples of 8, 16-byte aligned operands. */
 int hk_gemm_split(int parts, int m, int n, int k, const void *a, long long lda, const void *b, long long ldb,
                   double *out, long long ldo, void *stream);
+
+/* The same product from HK_F16X2 operands: h0 h0' + 2^-11 (h0 h1' + h1 h0') on the fp16 matrix
+ * cores (v_mfma_f32_32x32x16_f16), divided by the two row scales; 22 bits per operand against
+ * 16 for two bf16 parts, at the same rate. */
+int hk_gemm_f16x2(int m, int n, int k, const void *a, long long lda, const void *b, long long ldb, double *out,
+                  long long ldo, void *stream);
 
 #ifdef __cplusplus
 }

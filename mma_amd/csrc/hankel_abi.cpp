@@ -113,7 +113,9 @@ int source_q_max(const hk_planes *p, const HostGrids &g, double *q_max)
     return HK_OK;
 }
 
-int operand_parts(int type) { return type == HK_BF16X2 ? 2 : type == HK_BF16X3 ? 3 : 1; }
+int operand_parts(int type) { return type == HK_BF16X2 || type == HK_F16X2 ? 2 : type == HK_BF16X3 ? 3 : 1; }
+// elements of the type an operand row needs at least (HK_F16X2: the parts and the row scale)
+long long operand_row(int type, int nq) { return (long long)operand_parts(type) * nq + (type == HK_F16X2 ? 4 : 0); }
 
 // r_j = r_0 + j dr to within a few ulps of max |r| (then angles advanced by rotation stay within
 // ~1e-13 rad of the exact ones)
@@ -262,8 +264,9 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
     if (q_max / h > (double)(nq - HK_GRID_P - 1) * (1.0 + 1e-9))
         return fail(HK_ERR_ARGUMENT, "q grid too short for these sources (use hk_grid_size)");
     hipStream_t s = (hipStream_t)stream;
-    if (type < HK_F64 || type > HK_BF16X3) return fail(HK_ERR_ARGUMENT, "unknown operand type");
-    if (ld < (long long)operand_parts(type) * nq) return fail(HK_ERR_ARGUMENT, "row stride below the operand's parts x nq");
+    if (type < HK_F64 || type > HK_F16X2) return fail(HK_ERR_ARGUMENT, "unknown operand type");
+    if (ld < operand_row(type, nq) || (type == HK_F16X2 && ld % 4 != 0))
+        return fail(HK_ERR_ARGUMENT, "row stride below what a row of this operand type holds");
     // bins, then (when they fit the LDS too) the per-plane factors of the register-window kernel
     const size_t lds_bins = (size_t)nq * sizeof(double2);
     const size_t lds_tab = lds_bins + (size_t)p->nz * kSpreadTab * sizeof(double);
@@ -296,12 +299,14 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
         st = type == HK_F64   ? by_src(std::integral_constant<int, HK_F64>{})
              : type == HK_F32 ? by_src(std::integral_constant<int, HK_F32>{})
              : type == HK_BF16X2 ? by_src(std::integral_constant<int, HK_BF16X2>{})
-                                 : by_src(std::integral_constant<int, HK_BF16X3>{});
+             : type == HK_BF16X3 ? by_src(std::integral_constant<int, HK_BF16X3>{})
+                                 : by_src(std::integral_constant<int, HK_F16X2>{});
     } else {
         st = type == HK_F64   ? launch(hk_spread_kernel<HK_F64>)
              : type == HK_F32 ? launch(hk_spread_kernel<HK_F32>)
              : type == HK_BF16X2 ? launch(hk_spread_kernel<HK_BF16X2>)
-                                 : launch(hk_spread_kernel<HK_BF16X3>);
+             : type == HK_BF16X3 ? launch(hk_spread_kernel<HK_BF16X3>)
+                                 : launch(hk_spread_kernel<HK_F16X2>);
     }
     if (st != HK_OK) return st;
     HK_HIP(hipGetLastError());
@@ -311,8 +316,8 @@ int hk_grid_spread(const hk_planes *p, int n_out, const double *weights, int nq,
 int hk_grid_bessel(int nrho, const double *rho, int nq, double h, int type, void *out, long long ld, void *stream)
 {
     clear_status();
-    if (nrho < 1 || !rho || nq < 1 || !out || !(h > 0) || type < HK_F64 || type > HK_BF16X3 ||
-        ld < (long long)operand_parts(type) * nq)
+    if (nrho < 1 || !rho || nq < 1 || !out || !(h > 0) || type < HK_F64 || type > HK_F16X2 ||
+        ld < operand_row(type, nq) || (type == HK_F16X2 && ld % 4 != 0))
         return fail(HK_ERR_ARGUMENT, "bad Bessel arguments");
     hipStream_t s = (hipStream_t)stream;
     const long long n = (long long)nrho * nq;
@@ -323,8 +328,10 @@ int hk_grid_bessel(int nrho, const double *rho, int nq, double h, int type, void
         hipLaunchKernelGGL(hk_bessel_kernel<HK_F32>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
     else if (type == HK_BF16X2)
         hipLaunchKernelGGL(hk_bessel_kernel<HK_BF16X2>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
-    else
+    else if (type == HK_BF16X3)
         hipLaunchKernelGGL(hk_bessel_kernel<HK_BF16X3>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
+    else
+        hipLaunchKernelGGL(hk_bessel_kernel<HK_F16X2>, grid, dim3(256), 0, s, nrho, rho, nq, h, out, ld);
     HK_HIP(hipGetLastError());
     return HK_OK;
 }
@@ -370,6 +377,25 @@ int hk_gemm_split(int parts, int m, int n, int k, const void *a, long long lda, 
     };
     const int st = parts == 2 ? launch(hk_gemm_split_kernel<2>) : launch(hk_gemm_split_kernel<3>);
     if (st != HK_OK) return st;
+    HK_HIP(hipGetLastError());
+    return HK_OK;
+}
+
+int hk_gemm_f16x2(int m, int n, int k, const void *a, long long lda, const void *b, long long ldb, double *out,
+                  long long ldo, void *stream)
+{
+    clear_status();
+    if (m < 1 || n < 1 || k < 64 || k % 64 != 0 || !a || !b || !out || lda < 2LL * k + 4 || ldb < 2LL * k + 4 ||
+        ldo < n || lda % 8 != 0 || ldb % 8 != 0 || ((uintptr_t)a & 15) != 0 || ((uintptr_t)b & 15) != 0)
+        return fail(HK_ERR_ARGUMENT, "fp16 GEMM needs k a multiple of 64, strides >= 2 k + 4 and multiples of 8, "
+                                     "16-byte aligned operands");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)((n + 127) / 128), (unsigned)((m + 127) / 128));
+    const size_t lds = hk_gemm_split_lds(2);
+    auto kernel = hk_gemm_f16x2_kernel;
+    HK_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kernel, grid, dim3(512), lds, s, m, n, k, static_cast<const unsigned short *>(a), lda,
+                       static_cast<const unsigned short *>(b), ldb, out, ldo);
     HK_HIP(hipGetLastError());
     return HK_OK;
 }

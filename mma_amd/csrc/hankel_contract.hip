@@ -205,15 +205,24 @@ struct Lagrange {
     }
 };
 
-// element l of an operand row starting at `row` (ld already applied): double, float, or the
-// value split into 2 or 3 bf16 parts v ~ p0 + p1 (+ p2), part q at row + q nq + l
+// element l of an operand row starting at `row` (ld already applied): double, float, the value
+// split into 2 or 3 bf16 parts v ~ p0 + p1 (+ p2) (part q at row + q nq + l), or, for HK_F16X2,
+// v * scale split into two fp16 parts h0 + 2^-11 h1 (the row's scale, a power of two that puts
+// its largest element in [2^13, 2^14), sits as a double at row + 2 nq)
 template <int TYPE>
-__device__ inline void store_operand(void *out, long long row, int l, int nq, double v)
+__device__ inline void store_operand(void *out, long long row, int l, int nq, double v, double scale = 1.0)
 {
     if constexpr (TYPE == HK_F64) {
         reinterpret_cast<double *>(out)[row + l] = v;
     } else if constexpr (TYPE == HK_F32) {
         reinterpret_cast<float *>(out)[row + l] = (float)v;
+    } else if constexpr (TYPE == HK_F16X2) {
+        unsigned short *o = reinterpret_cast<unsigned short *>(out) + row + l;
+        const double x = v * scale;
+        const _Float16 h0 = (_Float16)(float)x;
+        const _Float16 h1 = (_Float16)(float)((x - (double)h0) * 2048.0);
+        o[0] = __builtin_bit_cast(unsigned short, h0);
+        o[nq] = __builtin_bit_cast(unsigned short, h1);
     } else {
         constexpr int NP = TYPE == HK_BF16X2 ? 2 : 3;
         unsigned short *o = reinterpret_cast<unsigned short *>(out) + row + l;
@@ -226,6 +235,9 @@ __device__ inline void store_operand(void *out, long long row, int l, int nq, do
         }
     }
 }
+
+// the power of two that puts max in [2^13, 2^14) (1 for a zero row)
+__device__ inline double f16_row_scale(double max) { return max > 0.0 ? ldexp(1.0, 13 - ilogb(max)) : 1.0; }
 
 // Adds the Lagrange-weighted item v at grid position t to the bins (real and imaginary parts
 // in separate LDS arrays: fewer bank conflicts than interleaved pairs).
@@ -246,9 +258,45 @@ template <int TYPE>
 __device__ inline void store_bins(const double *acc_re, const double *acc_im, int nq, void *out, long long ld,
                                   long long re_row)
 {
+    double s_re = 1.0, s_im = 1.0;
+    if constexpr (TYPE == HK_F16X2) {  // every thread of the workgroup takes part
+        __shared__ double part[2][kSpreadThreads / 64];
+        __shared__ double scale[2];
+        double m_re = 0.0, m_im = 0.0;
+        for (int l = threadIdx.x; l < nq; l += blockDim.x) {
+            m_re = fmax(m_re, fabs(acc_re[l]));
+            m_im = fmax(m_im, fabs(acc_im[l]));
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            m_re = fmax(m_re, __shfl_xor(m_re, d, 64));
+            m_im = fmax(m_im, __shfl_xor(m_im, d, 64));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            part[0][threadIdx.x >> 6] = m_re;
+            part[1][threadIdx.x >> 6] = m_im;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0.0, b = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+                a = fmax(a, part[0][w]);
+                b = fmax(b, part[1][w]);
+            }
+            scale[0] = f16_row_scale(a);
+            scale[1] = f16_row_scale(b);
+            double *o = reinterpret_cast<double *>(reinterpret_cast<unsigned short *>(out) + re_row * ld + 2 * nq);
+            o[0] = scale[0];
+            o = reinterpret_cast<double *>(reinterpret_cast<unsigned short *>(out) + (re_row + 1) * ld + 2 * nq);
+            o[0] = scale[1];
+        }
+        __syncthreads();
+        s_re = scale[0];
+        s_im = scale[1];
+    }
     for (int l = threadIdx.x; l < nq; l += blockDim.x) {
-        store_operand<TYPE>(out, re_row * ld, l, nq, acc_re[l]);
-        store_operand<TYPE>(out, (re_row + 1) * ld, l, nq, acc_im[l]);
+        store_operand<TYPE>(out, re_row * ld, l, nq, acc_re[l], s_re);
+        store_operand<TYPE>(out, (re_row + 1) * ld, l, nq, acc_im[l], s_im);
     }
 }
 
@@ -319,6 +367,8 @@ template __global__ void hk_spread_kernel<HK_BF16X2>(hk_planes, const double *, 
                                                      long long);
 template __global__ void hk_spread_kernel<HK_BF16X3>(hk_planes, const double *, int, double, double, void *, long long,
                                                      long long);
+template __global__ void hk_spread_kernel<HK_F16X2>(hk_planes, const double *, int, double, double, void *, long long,
+                                                    long long);
 
 // The same spread, accumulated in registers.  A thread owns one radius r_j (and a contiguous
 // range of planes) and walks its planes in order.  Its items' bins l0 = floor(K r_j / (D_k h)) + 1
@@ -580,6 +630,12 @@ HK_SPREAD_WINDOW(HK_BF16X3, double2, 2)
 HK_SPREAD_WINDOW(HK_BF16X3, float2, 0)
 HK_SPREAD_WINDOW(HK_BF16X3, float2, 1)
 HK_SPREAD_WINDOW(HK_BF16X3, float2, 2)
+HK_SPREAD_WINDOW(HK_F16X2, double2, 0)
+HK_SPREAD_WINDOW(HK_F16X2, double2, 1)
+HK_SPREAD_WINDOW(HK_F16X2, double2, 2)
+HK_SPREAD_WINDOW(HK_F16X2, float2, 0)
+HK_SPREAD_WINDOW(HK_F16X2, float2, 1)
+HK_SPREAD_WINDOW(HK_F16X2, float2, 2)
 #undef HK_SPREAD_WINDOW
 
 // J0(rho_m |Q_l|), Q_l = (l - P/2) h
@@ -590,10 +646,17 @@ __global__ void hk_bessel_kernel(int nrho, const double *rho, int nq, double h, 
     if (idx >= (long long)nrho * nq) return;
     const int m = (int)(idx / nq), l = (int)(idx % nq);
     const double q = (double)(l - HK_GRID_P / 2) * h;
-    store_operand<TYPE>(out, (long long)m * ld, l, nq, hk::j0(rho[m] * q));
+    if constexpr (TYPE == HK_F16X2) {  // |J0| <= 1: one scale for every row
+        const double scale = 8192.0;
+        if (l == 0) *reinterpret_cast<double *>(reinterpret_cast<unsigned short *>(out) + (long long)m * ld + 2 * nq) = scale;
+        store_operand<TYPE>(out, (long long)m * ld, l, nq, hk::j0(rho[m] * q), scale);
+    } else {
+        store_operand<TYPE>(out, (long long)m * ld, l, nq, hk::j0(rho[m] * q));
+    }
 }
 
 template __global__ void hk_bessel_kernel<HK_F64>(int, const double *, int, double, void *, long long);
 template __global__ void hk_bessel_kernel<HK_F32>(int, const double *, int, double, void *, long long);
 template __global__ void hk_bessel_kernel<HK_BF16X2>(int, const double *, int, double, void *, long long);
 template __global__ void hk_bessel_kernel<HK_BF16X3>(int, const double *, int, double, void *, long long);
+template __global__ void hk_bessel_kernel<HK_F16X2>(int, const double *, int, double, void *, long long);

@@ -199,12 +199,16 @@ struct SplitShape {
     static constexpr int FOLD = 64 / SBK;               // stages per FP64 fold (64 K terms)
 };
 
-template <int NP>
-__global__ void __launch_bounds__(kThreads, 1)
-hk_gemm_split_kernel(int M, int N, int K, const unsigned short *A, long long lda, const unsigned short *B,
-                     long long ldb, double *out, long long ldo)
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// F16: the two parts are fp16 (HK_F16X2): h0 h0' on one set of MFMA sums, h0 h1' + h1 h0' on a
+// second one folded in with 2^-11, and the result divided by the rows' scales.
+template <int NP, bool F16>
+__device__ inline void split_gemm(int M, int N, int K, const unsigned short *A, long long lda, const unsigned short *B,
+                                  long long ldb, double *out, long long ldo)
 {
     using S = SplitShape<NP>;
+    using op8 = typename std::conditional<F16, f16x8, bf16x8>::type;
     extern __shared__ __attribute__((aligned(16))) unsigned short lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;  // rows 32 wm .. +32, columns 64 wn .. +64
@@ -230,7 +234,7 @@ hk_gemm_split_kernel(int M, int N, int K, const unsigned short *A, long long lda
     };
     auto operand = [&](int buf, int q, int row, int c) {
         const int slot = c ^ ((row / S::RPB) & (S::CH - 1));
-        return *reinterpret_cast<const bf16x8 *>(lds + buf * S::TILE + (q * S::ROWS + row) * S::SBK + 8 * slot);
+        return *reinterpret_cast<const op8 *>(lds + buf * S::TILE + (q * S::ROWS + row) * S::SBK + 8 * slot);
     };
 
     double accd[2][16];
@@ -238,11 +242,14 @@ hk_gemm_split_kernel(int M, int N, int K, const unsigned short *A, long long lda
     for (int tj = 0; tj < 2; tj++)
 #pragma unroll
         for (int v = 0; v < 16; v++) accd[tj][v] = 0.0;
-    f32x16 acc[2];
+    f32x16 acc[2], acc1[2];  // acc1: the fp16 cross terms
 #pragma unroll
     for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-        for (int v = 0; v < 16; v++) acc[tj][v] = 0.0f;
+        for (int v = 0; v < 16; v++) {
+            acc[tj][v] = 0.0f;
+            acc1[tj][v] = 0.0f;
+        }
 
     const int stages = K / S::SBK;
     issue(0, 0);
@@ -262,12 +269,23 @@ hk_gemm_split_kernel(int M, int N, int K, const unsigned short *A, long long lda
 #pragma unroll
         for (int kk = 0; kk < S::SBK; kk += 16) {
             const int c = kk / 8 + kh;
-            bf16x8 ap[NP], bp[NP][2];
+            op8 ap[NP], bp[NP][2];
 #pragma unroll
             for (int q = 0; q < NP; q++) {
                 ap[q] = operand(buf, q, wm * 32 + r32, c);
 #pragma unroll
                 for (int t = 0; t < 2; t++) bp[q][t] = operand(buf, q, BM + wn * 64 + t * 32 + r32, c);
+            }
+            if constexpr (F16) {
+#pragma unroll
+                for (int tj = 0; tj < 2; tj++) {
+                    f32x16 x = acc1[tj];
+                    x = __builtin_amdgcn_mfma_f32_32x32x16_f16(ap[1], bp[0][tj], x, 0, 0, 0);
+                    x = __builtin_amdgcn_mfma_f32_32x32x16_f16(ap[0], bp[1][tj], x, 0, 0, 0);
+                    acc1[tj] = x;
+                    acc[tj] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ap[0], bp[0][tj], acc[tj], 0, 0, 0);
+                }
+                continue;
             }
 #pragma unroll
             for (int tj = 0; tj < 2; tj++) {
@@ -290,18 +308,45 @@ hk_gemm_split_kernel(int M, int N, int K, const unsigned short *A, long long lda
                 for (int v = 0; v < 16; v++) {
                     accd[tj][v] += (double)acc[tj][v];
                     acc[tj][v] = 0.0f;
+                    if constexpr (F16) {
+                        accd[tj][v] += 0x1p-11 * (double)acc1[tj][v];
+                        acc1[tj][v] = 0.0f;
+                    }
                 }
         }
         __builtin_amdgcn_s_barrier();  // buffer `buf` is free for stage s + 2
     }
 #pragma unroll
-    for (int tj = 0; tj < 2; tj++)
+    for (int tj = 0; tj < 2; tj++) {
+        const long long col = n0 + wn * 64 + tj * 32 + r32;
+        double inv_b = 1.0;
+        if constexpr (F16)
+            if (col < N) inv_b = 1.0 / *reinterpret_cast<const double *>(B + col * ldb + 2LL * K);
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const long long row = m0 + wm * 32 + 8 * (v >> 2) + 4 * kh + (v & 3);
-            const long long col = n0 + wn * 64 + tj * 32 + r32;
-            if (row < M && col < N) out[row * ldo + col] = accd[tj][v];
+            if (row < M && col < N) {
+                double r = accd[tj][v];
+                if constexpr (F16) r = r * inv_b / *reinterpret_cast<const double *>(A + row * lda + 2LL * K);
+                out[row * ldo + col] = r;
+            }
         }
+    }
+}
+
+template <int NP>
+__global__ void __launch_bounds__(kThreads, 1)
+hk_gemm_split_kernel(int M, int N, int K, const unsigned short *A, long long lda, const unsigned short *B,
+                     long long ldb, double *out, long long ldo)
+{
+    split_gemm<NP, false>(M, N, K, A, lda, B, ldb, out, ldo);
+}
+
+__global__ void __launch_bounds__(kThreads, 1)
+hk_gemm_f16x2_kernel(int M, int N, int K, const unsigned short *A, long long lda, const unsigned short *B,
+                     long long ldb, double *out, long long ldo)
+{
+    split_gemm<2, true>(M, N, K, A, lda, B, ldb, out, ldo);
 }
 
 template __global__ void hk_gemm_split_kernel<2>(int, int, int, const unsigned short *, long long,

@@ -47,7 +47,9 @@ __global__ void hk_bessel_kernel(int nrho, const double *rho, int nq, double h, 
 template <int NP>
 __global__ void hk_gemm_split_kernel(int M, int N, int K, const unsigned short *A, long long lda, const unsigned short *B,
                                      long long ldb, double *out, long long ldo);
-size_t hk_gemm_split_lds(int np);  // dynamic LDS of hk_gemm_split_kernel<np>
+size_t hk_gemm_split_lds(int np);  // dynamic LDS of hk_gemm_split_kernel<np> (and of the f16x2 kernel: np 2)
+__global__ void hk_gemm_f16x2_kernel(int M, int N, int K, const unsigned short *A, long long lda, const unsigned short *B,
+                                     long long ldb, double *out, long long ldo);
 
 template <int KIND>
 __global__ void hk_gemm_kernel(int m, int n, int k, const float *a, long long lda, const float *b, long long ldb,

@@ -13,8 +13,10 @@ the source planes, and the gridded contraction on the matrix cores.
   'fp32-mfma'    the gridded form with the GEMM in FP32 on the matrix cores;
   'bf16x3-mfma'  the gridded form with the GEMM as three bf16 products (hi*hi + hi*lo + lo*hi)
                  on the matrix cores;
-  'bf16x6-mfma'  the same with a three-way bf16 split and six products.
-The matrix-core GEMMs accumulate 32-deep stages in FP32 and the stages in FP64.
+  'bf16x6-mfma'  the same with a three-way bf16 split and six products;
+  'fp16x3-mfma'  each operand row scaled by a power of two and split into two fp16 parts
+                 (22 bits), three fp16 products on the matrix cores.
+The matrix-core GEMMs accumulate 32- or 64-deep stages in FP32 and the stages in FP64.
 Only the trapezoidal radial integrator is accelerated; any other integrator raises
 NotImplementedError (the reference accepts an arbitrary callable).
 """
@@ -33,10 +35,10 @@ TABLES = os.path.join(HERE, "data", "xuv_tables.npz")
 C_LIGHT = 299792458.0  # shared_python/units.py:3
 
 HK_OK, HK_ERR_ARGUMENT, HK_ERR_UNSUPPORTED, HK_ERR_DEVICE, HK_ERR_TABLE = 0, 3, 4, 5, 7
-HK_F64, HK_F32, HK_BF16X2, HK_BF16X3 = 0, 1, 2, 3
+HK_F64, HK_F32, HK_BF16X2, HK_BF16X3, HK_F16X2 = 0, 1, 2, 3, 4
 HK_GEMM_F32, HK_GEMM_BF16X3, HK_GEMM_BF16X6 = 1, 2, 3
 GRID_TOL = 1e-10  # bound on the J0 interpolation error of the gridded contraction
-CONTRACTIONS = ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma")
+CONTRACTIONS = ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma", "fp16x3-mfma")
 
 
 class hk_medium(Structure):
@@ -74,6 +76,8 @@ def _L():
         L.hk_gemm.argtypes = [c_int, c_int, c_int, c_int, c_void_p, c_longlong, c_void_p, c_longlong, c_void_p,
                               c_longlong, c_void_p]
         L.hk_gemm_split.argtypes = [c_int, c_int, c_int, c_int, c_void_p, c_longlong, c_void_p, c_longlong, c_void_p,
+                                    c_longlong, c_void_p]
+        L.hk_gemm_f16x2.argtypes = [c_int, c_int, c_int, c_void_p, c_longlong, c_void_p, c_longlong, c_void_p,
                                     c_longlong, c_void_p]
         _lib = L
     return _lib
@@ -426,23 +430,29 @@ def gridded(src, zgrid, ogrid, rgrid, rho, distance, weights, near_field=True, p
     nq, h = grid_size_of(zgrid, ogrid, rgrid, rho, distance, tol)
     W = np.atleast_2d(np.asarray(weights, dtype=np.float64))
     n_out = W.shape[0]
-    # operands: FP64 (library GEMM), FP32 (FP32 MFMA) or pre-split bf16 parts (bf16 MFMA)
-    parts = {"bf16x3-mfma": 2, "bf16x6-mfma": 3}.get(kind, 1)
-    otype = {"fp64-grid": HK_F64, "fp32-mfma": HK_F32, "bf16x3-mfma": HK_BF16X2, "bf16x6-mfma": HK_BF16X3}[kind]
+    # operands: FP64 (library GEMM), FP32 (FP32 MFMA), pre-split bf16 parts (bf16 MFMA) or scaled
+    # fp16 parts with their row scale (fp16 MFMA)
+    otype = {"fp64-grid": HK_F64, "fp32-mfma": HK_F32, "bf16x3-mfma": HK_BF16X2, "bf16x6-mfma": HK_BF16X3,
+             "fp16x3-mfma": HK_F16X2}[kind]
+    parts = {HK_BF16X2: 2, HK_BF16X3: 3, HK_F16X2: 2}.get(otype, 1)
+    ld = parts * nq + (8 if otype == HK_F16X2 else 0)  # row: the parts (and the scale, 4 words padded to 8)
     dt = {HK_F64: torch.float64, HK_F32: torch.float32}.get(otype, torch.int16)
-    Bm = torch.empty((n_out * 2 * p.no, parts * nq), dtype=dt, device="cuda")
+    Bm = torch.empty((n_out * 2 * p.no, ld), dtype=dt, device="cuda")
     wq = _dev(np.ascontiguousarray(W))
-    _check(L.hk_grid_spread(byref(p), n_out, _ptr(wq), nq, h, otype, _ptr(Bm), parts * nq, 0, _stream()))
+    _check(L.hk_grid_spread(byref(p), n_out, _ptr(wq), nq, h, otype, _ptr(Bm), ld, 0, _stream()))
     rh = _dev(rho)
-    Am = torch.empty((p.nrho, parts * nq), dtype=dt, device="cuda")
-    _check(L.hk_grid_bessel(p.nrho, _ptr(rh), nq, h, otype, _ptr(Am), parts * nq, _stream()))
+    Am = torch.empty((p.nrho, ld), dtype=dt, device="cuda")
+    _check(L.hk_grid_bessel(p.nrho, _ptr(rh), nq, h, otype, _ptr(Am), ld, _stream()))
     if kind == "fp64-grid":
         out = Am @ Bm.T  # plain FP64 library GEMM
     else:
         out = torch.empty((p.nrho, Bm.shape[0]), dtype=torch.float64, device="cuda")
-        if parts > 1:
-            _check(L.hk_gemm_split(parts, p.nrho, Bm.shape[0], nq, _ptr(Am), parts * nq, _ptr(Bm), parts * nq,
-                                   _ptr(out), Bm.shape[0], _stream()))
+        if otype == HK_F16X2:
+            _check(L.hk_gemm_f16x2(p.nrho, Bm.shape[0], nq, _ptr(Am), ld, _ptr(Bm), ld, _ptr(out), Bm.shape[0],
+                                   _stream()))
+        elif parts > 1:
+            _check(L.hk_gemm_split(parts, p.nrho, Bm.shape[0], nq, _ptr(Am), ld, _ptr(Bm), ld, _ptr(out),
+                                   Bm.shape[0], _stream()))
         else:
             _check(L.hk_gemm(HK_GEMM_F32, p.nrho, Bm.shape[0], nq, _ptr(Am), nq, _ptr(Bm), nq, _ptr(out), Bm.shape[0],
                              _stream()))

@@ -176,7 +176,7 @@ def test_hankel_long_fp64_matches_oracle():
 
 
 @pytest.mark.parametrize("contraction,tol", [("fp64-grid", 1e-9), ("fp32-mfma", 1e-6), ("bf16x3-mfma", 1e-6),
-                                             ("bf16x6-mfma", 1e-6)])
+                                             ("bf16x6-mfma", 1e-6), ("fp16x3-mfma", 1e-6)])
 def test_hankel_long_gridded(contraction, tol):
     from mma_amd import hankel as H
     zall, r, o, S, pressure, rho = _long_case()
@@ -290,6 +290,48 @@ def test_split_gemm_matches_fp64():
             assert e < 1e-6
 
 
+def test_f16x2_gemm_matches_fp64():
+    """hk_gemm_f16x2 on operands scaled and split on the host (each row by a power of two that puts
+    its largest element in [2^13, 2^14), then v s = h0 + 2^-11 h1 in fp16, the scale after the
+    parts): h0 h0' + 2^-11 (h0 h1' + h1 h0') over the two scales, against its FP64 sum, with
+    ragged m and n, rows of very different magnitude and a zero row."""
+    from mma_amd import hankel as H
+    rng = np.random.default_rng(12)
+
+    def split(X):
+        mx = np.max(np.abs(X), axis=1)
+        e = np.where(mx > 0, np.floor(np.log2(np.where(mx > 0, mx, 1.0))), 0.0)
+        s = np.where(mx > 0, 2.0 ** (13 - e), 1.0)
+        x = X * s[:, None]
+        h0 = x.astype(np.float32).astype(np.float16)
+        h1 = ((x - h0.astype(np.float64)) * 2048.0).astype(np.float32).astype(np.float16)
+        k = X.shape[1]
+        row = np.zeros((X.shape[0], 2 * k + 8), dtype=np.uint16)
+        row[:, :k] = h0.view(np.uint16)
+        row[:, k:2 * k] = h1.view(np.uint16)
+        row[:, 2 * k:2 * k + 4] = s.astype(np.float64).view(np.uint16).reshape(-1, 4)
+        return row, h0.astype(np.float64), h1.astype(np.float64), s
+
+    for m, n, k in ((200, 330, 256), (1024, 600, 5120)):
+        A = rng.standard_normal((m, k)) * 10.0 ** rng.uniform(-6, 6, (m, 1))
+        B = rng.standard_normal((n, k)) * 10.0 ** rng.uniform(-6, 6, (n, 1))
+        B[3] = 0.0
+        ra, a0, a1, sa = split(A)
+        rb, b0, b1, sb = split(B)
+        ref = (a0 @ b0.T + (a0 @ b1.T + a1 @ b0.T) / 2048.0) / np.outer(sa, sb)
+        Ad = torch.tensor(ra.view(np.int16), device="cuda")
+        Bd = torch.tensor(rb.view(np.int16), device="cuda")
+        out = torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda")
+        H._check(H._L().hk_gemm_f16x2(m, n, k, H._ptr(Ad), 2 * k + 8, H._ptr(Bd), 2 * k + 8, H._ptr(out), n,
+                                      H._stream()))
+        got = out.cpu().numpy()
+        assert np.all(got[:, 3] == 0.0)
+        # each column against its own scale (the columns' magnitudes span 12 decades)
+        e = max(rel_err(got[:, j], ref[:, j]) for j in range(n) if j != 3)
+        print("f16x2 gemm %dx%dx%d: %.2e" % (m, n, k, e), flush=True)
+        assert e < 1e-6
+
+
 def test_full_size_far_field_error_bound():
     """The far-field workload at full size (256 planes x 2048 frequencies x 512 radii -> 1024
     screen radii at 1 m, Ar cell with a z-modulated pressure, complex128 and complex64 sources)
@@ -313,9 +355,10 @@ def test_full_size_far_field_error_bound():
     names = ("FF_integrated", "entry_plane_transform", "exit_plane_transform")
     # (contraction, sources, bound on max |d| / max |ref|, bound on the Frobenius ratio)
     cases = [("fp64-grid", src, 1e-9, 1e-9), ("fp32-mfma", src, 1e-6, 1e-6), ("bf16x6-mfma", src, 1e-6, 1e-6),
-             ("bf16x3-mfma", src, 2e-6, 1e-6)]
+             ("fp16x3-mfma", src, 1e-6, 1e-6), ("bf16x3-mfma", src, 2e-6, 1e-6)]
     src64 = src.to(torch.complex64)
-    cases += [("fp32-mfma", src64, 1e-6, 1e-6), ("bf16x6-mfma", src64, 1e-6, 1e-6), ("bf16x3-mfma", src64, 2e-6, 1e-6)]
+    cases += [("fp32-mfma", src64, 1e-6, 1e-6), ("bf16x6-mfma", src64, 1e-6, 1e-6), ("fp16x3-mfma", src64, 1e-6, 1e-6),
+              ("bf16x3-mfma", src64, 2e-6, 1e-6)]
     for contraction, sources, tol, tol_f in cases:
         res = run(contraction, sources)
         for name in names:
