@@ -185,9 +185,13 @@ int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const 
         // the z-tabulated walk reads the plane grid and the pressure table from LDS when they fit
         const size_t stage = (size_t)(nz + 2 * m->np_z) * sizeof(double);
         const int staged = m->mode == HK_PRESSURE_Z && stage <= 48 * 1024;
-        hipLaunchKernelGGL(hk_prefactor_kernel, dim3((n + 127) / 128, planes), dim3(128), staged ? stage : 0, s,
-                           make_constants(), *m, nz, zgrid, no, ogrid, nr, rgrid, reinterpret_cast<double2 *>(pre),
-                           renorm, exit_abs, flag, staged);
+        if (m->mode == HK_PRESSURE_Z && m->np_z <= kWalkParallelMax && nz <= 65535)
+            hipLaunchKernelGGL(hk_prefactor_z_kernel, dim3((no + 127) / 128, nz), dim3(128), 0, s, make_constants(), *m,
+                               nz, zgrid, no, ogrid, reinterpret_cast<double2 *>(pre), renorm, exit_abs, flag);
+        else
+            hipLaunchKernelGGL(hk_prefactor_kernel, dim3((n + 127) / 128, planes), dim3(128), staged ? stage : 0, s,
+                               make_constants(), *m, nz, zgrid, no, ogrid, nr, rgrid, reinterpret_cast<double2 *>(pre),
+                               renorm, exit_abs, flag, staged);
         if (m->mode == HK_PRESSURE_Z) {
             const long long total = (long long)nz * no;
             hipLaunchKernelGGL(hk_prefactor_finish_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nz,

@@ -212,6 +212,95 @@ __global__ void hk_prefactor_kernel(HkConst K, hk_medium m, int nz, const double
     if (!zr) exit_abs[o] = last_abs;
 }
 
+// The z-tabulated branch with one thread per (frequency, plane) instead of one per frequency:
+// each thread repeats the table walk of hk_prefactor_kernel from the table's start up to its own
+// plane (the walk's state at a plane depends on that plane's z only), so the cumulative
+// trapezoids are formed in the same order and the stash holds the same bits; grid (ceil(no /
+// 128), nz).  For tables of moderate length (the host picks it for np <= kWalkParallelMax).
+__global__ void hk_prefactor_z_kernel(HkConst K, hk_medium m, int nz, const double *zgrid, int no, const double *ogrid,
+                                      double2 *pre, double *renorm, double *exit_abs, int *flag)
+{
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (o >= no) return;
+    const double w = ogrid[o];
+    const double w_au = w * K.timeau;
+    const double energy = w_au / K.ev_per_au;
+    const double lam = K.lam_num / (w_au * K.lam_den);
+    double f1 = 0.0, f2 = 0.0;
+    if (m.include_dispersion) {
+        if (!interp_checked(m.e_f1, m.f1, m.n_f1, energy, f1)) {
+            if (k == 0) atomicMax(flag, HK_FLAG_RANGE);
+            return;
+        }
+        if (f1 < -9998.0) {  // XUV_refractive_index.py:156
+            if (k == 0) atomicMax(flag, HK_FLAG_F1);
+            return;
+        }
+    }
+    if (m.include_absorption && !interp_checked(m.e_f2, m.f2, m.n_f2, energy, f2)) {
+        if (k == 0) atomicMax(flag, HK_FLAG_RANGE);
+        return;
+    }
+    const double c = K.c_light;
+    const double v_ir = c / m.n_ir;
+    auto disp = [&](double p) {
+        const double n_xuv = 1.0 - p * K.n_ref_re * ((lam * lam) * f1 / (2.0 * M_PI));
+        const double v_xuv = c / n_xuv;
+        return (1.0 / v_ir) - (1.0 / v_xuv);
+    };
+    const double beta = K.n_ref_re * ((lam * lam) * f2 / (2.0 * M_PI));
+    const double disp_vac = (m.n_ir - 1.0) / c;
+    const int np = m.np_z;
+    const double *zp = m.p_z;
+    Walk W;
+    double d_i = m.include_dispersion ? disp(m.p_value[0]) : 0.0, p_i = m.p_value[0];
+    double d_n = 0.0, p_n = p_i;
+    if (np > 1) {
+        p_n = m.p_value[1];
+        d_n = m.include_dispersion ? disp(p_n) : 0.0;
+        W.cd_n = ((zp[1] - zp[0]) * (d_i + d_n)) / 2.0;
+        W.cp_n = ((zp[1] - zp[0]) * (p_i + p_n)) / 2.0;
+    }
+    auto advance = [&]() {
+        W.i++;
+        W.cd_i = W.cd_n;
+        W.cp_i = W.cp_n;
+        d_i = d_n;
+        p_i = p_n;
+        p_n = m.p_value[W.i + 1];
+        d_n = m.include_dispersion ? disp(p_n) : 0.0;
+        W.cd_n = W.cd_i + ((zp[W.i + 1] - zp[W.i]) * (d_i + d_n)) / 2.0;
+        W.cp_n = W.cp_i + ((zp[W.i + 1] - zp[W.i]) * (p_i + p_n)) / 2.0;
+    };
+    const double z = zgrid[k];
+    double cd, cp, pm;
+    if (np == 1 || z < zp[0]) {
+        cd = 0.0;
+        cp = 0.0;
+        pm = m.p_value[0];
+    } else {
+        while (W.i < np - 2 && zp[W.i + 1] < z) advance();
+        if (z > zp[np - 1]) {
+            while (W.i < np - 2) advance();
+            cd = W.cd_n;
+            cp = W.cp_n;
+            pm = p_n;
+        } else {
+            const double h = zp[W.i + 1] - zp[W.i];
+            cd = ((W.cd_n - W.cd_i) / h) * (z - zp[W.i]) + W.cd_i;
+            cp = ((W.cp_n - W.cp_i) / h) * (z - zp[W.i]) + W.cp_i;
+            pm = ((p_n - p_i) / h) * (z - zp[W.i]) + p_i;
+        }
+    }
+    const double dphase = m.include_dispersion ? cd : disp_vac;
+    const double absf = m.include_absorption ? (1.0 / c) * (beta * cp) : 0.0;
+    const long long at = (long long)k * no + o;
+    pre[at] = make_double2(absf, dphase);
+    renorm[at] = pm;
+    if (k == nz - 1) exit_abs[o] = absf;
+}
+
 // The z-tabulated branch's exponentials from the stash (absorption, dispersion phase) in pre
 // and the pressure in renorm, renormalised to the exit plane (:215, :226-228).
 __global__ void hk_prefactor_finish_kernel(int nz, int no, const double *ogrid, const double *exit_abs, double2 *pre,

@@ -35,8 +35,11 @@ def test_bessel_j0_fp64():
         assert err < 3e-15
 
 
-@pytest.mark.parametrize("case", ["scalar", "z", "zr", "vacuum"])
+@pytest.mark.parametrize("case", ["scalar", "z", "z_long", "zr", "vacuum"])
 def test_prefactor_tables(case):
+    """Pre-factor tables against the oracle for every pressure branch; "z" runs the walk of the
+    pressure table with one thread per (frequency, plane), "z_long" (5000 table points, beyond
+    the length that form takes) with one thread per frequency."""
     from mma_amd import hankel as H
     z = np.linspace(0.0, 5e-3, 17)
     r = np.linspace(0.0, 1e-4, 9)
@@ -44,8 +47,8 @@ def test_prefactor_tables(case):
     kw = dict(absorption_tables="Henke", dispersion_tables="NIST")
     if case == "scalar":
         args = ("Ar", 0.6)
-    elif case == "z":
-        zp = np.linspace(-1e-3, 6e-3, 57)
+    elif case in ("z", "z_long"):
+        zp = np.linspace(-1e-3, 6e-3, 57 if case == "z" else 5000)
         args = ("Ar", {"zgrid": zp, "value": 0.3 + 0.7 * np.exp(-((zp - 2e-3) / 1.5e-3) ** 2)})
     elif case == "zr":
         zp = np.linspace(0.0, 5e-3, 23)
