@@ -50,19 +50,22 @@ __device__ inline double2 cmul(double2 a, double2 b)
 // ---- cross-lane moves in the VALU (DPP), 64-bit values as two 32-bit halves ----
 // Lanes without a source keep `old` (bound_ctrl off), so scans pad with the identity map.
 constexpr int kRowShr = 0x110, kRowShl = 0x100, kWaveShr1 = 0x138, kWaveShl1 = 0x130;
+// lane 15 of each row to the next row, lane 31 to rows 2 and 3 (with the row masks below)
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
 
-template <int CTRL>
+// ROWS: the rows (of 16 lanes) that take the moved value; the others keep `old`
+template <int CTRL, int ROWS = 0xf>
 __device__ inline double dpp(double v, double old)
 {
     const long long b = __builtin_bit_cast(long long, v), o = __builtin_bit_cast(long long, old);
-    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, ROWS, 0xf, false);
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
-template <int CTRL>
+template <int CTRL, int ROWS = 0xf>
 __device__ inline double2 dpp2(double2 v, double2 old)
 {
-    return make_double2(dpp<CTRL>(v.x, old.x), dpp<CTRL>(v.y, old.y));
+    return make_double2(dpp<CTRL, ROWS>(v.x, old.x), dpp<CTRL, ROWS>(v.y, old.y));
 }
 __device__ inline double readlane(double v, int l)
 {
@@ -81,7 +84,9 @@ __device__ inline void compose(double2 &Q, double2 &z, double2 Qo, double2 zo)
 
 // Inclusive scan over the 64 lanes of the affine maps x_out = z + Q x_in, in lane order
 // (FWD: lane l composes lanes 0..l) or reverse order (lanes l..63).  Rows of 16 lanes
-// scan with DPP shifts; rows are then joined through the row totals read with readlane.
+// scan with DPP shifts; rows are then joined: in lane order by two DPP row broadcasts (lane 15
+// into rows 1 and 3, then lane 31 into rows 2 and 3), in reverse order through the row totals
+// read with readlane.
 template <bool FWD>
 __device__ inline void wave_scan(double2 &Q, double2 &z, int lane)
 {
@@ -97,36 +102,37 @@ __device__ inline void wave_scan(double2 &Q, double2 &z, int lane)
     TDSE_SCAN_LEVEL(4)
     TDSE_SCAN_LEVEL(8)
 #undef TDSE_SCAN_LEVEL
-    // row totals: last lane of each row (forward) or first lane (reverse)
+    if constexpr (FWD) {
+        // rows 1 and 3 take the totals of rows 0 and 2; then rows 2 and 3 take lanes 0..31
+        // (rows that take nothing compose with the identity map, which leaves them exact)
+        {
+            const double2 Qo = dpp2<kRowBcast15, 0xa>(Q, one), zo = dpp2<kRowBcast15, 0xa>(z, zero);
+            compose(Q, z, Qo, zo);
+        }
+        {
+            const double2 Qo = dpp2<kRowBcast31, 0xc>(Q, one), zo = dpp2<kRowBcast31, 0xc>(z, zero);
+            compose(Q, z, Qo, zo);
+        }
+        return;
+    }
+    // reverse order: row totals are the first lane of each row; carry into this row from the
+    // rows after it
     const int row = lane >> 4;
     double2 tq[4], tz[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int src = FWD ? 16 * r + 15 : 16 * r;
-        tq[r] = readlane2(Q, src);
-        tz[r] = readlane2(z, src);
+    for (int r = 1; r < 4; r++) {
+        tq[r] = readlane2(Q, 16 * r);
+        tz[r] = readlane2(z, 16 * r);
     }
-    // carry into this row from the rows before it in scan order
     double2 CQ = one, CZ = zero;
-    if (FWD) {
 #pragma unroll
-        for (int r = 0; r < 3; r++)
-            if (r < row) {
-                double2 q = tq[r], zz = tz[r];
-                compose(q, zz, CQ, CZ);
-                CQ = q;
-                CZ = zz;
-            }
-    } else {
-#pragma unroll
-        for (int r = 3; r > 0; r--)
-            if (r > row) {
-                double2 q = tq[r], zz = tz[r];
-                compose(q, zz, CQ, CZ);
-                CQ = q;
-                CZ = zz;
-            }
-    }
+    for (int r = 3; r > 0; r--)
+        if (r > row) {
+            double2 q = tq[r], zz = tz[r];
+            compose(q, zz, CQ, CZ);
+            CQ = q;
+            CZ = zz;
+        }
     compose(Q, z, CQ, CZ);
 }
 
