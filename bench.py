@@ -311,7 +311,8 @@ def hankel_measure(cpu=True):
     """Far-field workload: 256 source planes (5 mm Ar cell, z-modulated pressure, NIST f1 /
     Henke f2) x 2048 frequencies (harmonics 11-61 of 800 nm) x 512 radii (0-200 um) onto 1024
     screen radii (0-7 mm) at 1 m.  Times the device work of each contraction (pre-factor tables
-    included), and states every result's error against the FP64 direct contraction."""
+    included) on complex128 sources, and the matrix-core ones on complex64 sources too, and states
+    every result's error against the FP64 direct contraction."""
     import torch
     from mma_amd import hankel as H, workloads as W
     zall, rg, og, rho, src, distance = W.hankel_workload()
@@ -325,11 +326,11 @@ def hankel_measure(cpu=True):
     w[1:] += 0.5 * dz
     Wt = np.stack([w, np.eye(nz)[0], np.eye(nz)[-1]])
 
-    def run(kind):
+    def run(kind, sources=src):
         tab = H._PreFactor(zg, rg, og, "Ar", pressure, "Henke", True, "NIST", True, 1.0)
         if kind == "fp64":
-            return H.direct(src, zg, og, rg, rho, distance, weights=Wt, pre=tab.pre)
-        return H.gridded(src, zg, og, rg, rho, distance, Wt, pre=tab.pre, kind=kind)
+            return H.direct(sources, zg, og, rg, rho, distance, weights=Wt, pre=tab.pre)
+        return H.gridded(sources, zg, og, rg, rho, distance, Wt, pre=tab.pre, kind=kind)
 
     out, res = {}, {}
     for kind in ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma", "fp16x3-mfma"):
@@ -343,6 +344,20 @@ def hankel_measure(cpu=True):
         out[kind] = (time.perf_counter() - t0) / reps
         res[kind] = r
         log("hankel %s: %.2f ms" % (kind, out[kind] * 1e3))
+    # the same passes on complex64 sources (half the bytes for the spread to read)
+    src64 = src.to(torch.complex64)
+    out64, res64 = {}, {}
+    for kind in ("fp32-mfma", "bf16x3-mfma", "bf16x6-mfma", "fp16x3-mfma"):
+        run(kind, src64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            r = run(kind, src64)
+        torch.cuda.synchronize()
+        out64[kind] = (time.perf_counter() - t0) / 5
+        res64[kind] = r
+        log("hankel %s, complex64 sources: %.2f ms" % (kind, out64[kind] * 1e3))
+    del src64
     ref = res["fp64"]
     errors = {}
     # each output against its own scale: the integrated field (a sum over planes weighted by the
@@ -353,6 +368,12 @@ def hankel_measure(cpu=True):
             d = res[kind][i] - ref[i]
             errors[kind][name] = {"max_abs_over_max": float(d.abs().max() / ref[i].abs().max()),
                                   "frobenius": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref[i]))}
+    errors64 = {}
+    for kind in out64:
+        d = res64[kind][0] - ref[0]
+        errors64[kind] = {"FF_integrated": {
+            "max_abs_over_max": float(d.abs().max() / ref[0].abs().max()),
+            "frobenius": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref[0]))}}
     nq = int(H.grid_size_of(zg, og, rg, rho, distance)[0])
     gemm_flops = 2.0 * len(rho) * (3 * 2 * no) * nq
     info = {
@@ -362,6 +383,8 @@ def hankel_measure(cpu=True):
         "seconds": out,
         "errors_vs_fp64_direct": errors,
         "speedup_vs_fp64_direct": {k: out["fp64"] / out[k] for k in out if k != "fp64"},
+        "seconds_complex64_sources": out64,
+        "errors_complex64_sources_vs_fp64_direct": errors64,
         "fp64_direct_j0_per_s": elems / out["fp64"],
         "fp64_direct_algorithmic_tflops": 4.0 * elems / out["fp64"] / 1e12,
         "grid_points_q": nq,
