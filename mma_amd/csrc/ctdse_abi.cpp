@@ -102,6 +102,7 @@ struct ctdse_geometry {
     int gs_iters = 0;
     DevMem x, psi0_nat, g, dpot, ib, psi0, chq, chg, adq, adg;
     std::shared_ptr<Lattice> lattice;  // null, or the batch's shared tables this geometry reads
+    int lattice_shift = -1;            // the alignment of `lattice` it reads (0..7)
 };
 
 namespace {
@@ -430,6 +431,7 @@ int share_lattices(const std::vector<ctdse_geometry *> &gs)
             const int off = (big->num_r - G->num_r) / 2;
             const int shift = (Lattice::kAlign - off % Lattice::kAlign) % Lattice::kAlign;
             G->lattice = L;
+            G->lattice_shift = shift;
             G->dev.sg = L->g[shift].as<double>();
             G->dev.sdp = L->dpot[shift].as<double>();
             G->dev.sib = L->ib[shift].as<double2>();
@@ -736,6 +738,7 @@ struct ctdse_batch_plan {
     LaunchShape shape;
     long long stride = 0;
     DevMem work, d_geom, d_inst, d_order, d_counter;
+    TdseQueues queues{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t done = nullptr;  // recorded after every run outside a capture
@@ -754,12 +757,13 @@ struct ctdse_batch_plan {
 namespace {
 
 // Descriptors of a batch on the host: validated instances, distinct geometries and the
-// longest-first queue order.
+// per-XCD queues of instances, each in longest-first order.
 struct BatchHost {
     std::vector<const ctdse_geometry *> geoms;
     std::vector<TdseGeom> gdev;
     std::vector<TdseInst> insts;
     std::vector<int> order;
+    TdseQueues queues{};
     long long stride = 0;
 };
 
@@ -801,13 +805,29 @@ int describe(const std::vector<InstSpec> &spec, BatchHost &B)
         I.psi_final = s.psi_final;
         B.stride = std::max(B.stride, s.geom->dev.npad);
     }
-    // largest instances first (longest-processing-time order for the atomic queue)
-    B.order.resize(n);
-    for (int i = 0; i < n; i++) B.order[i] = i;
+    // largest instances first (longest-processing-time order for the atomic queues)
+    std::vector<int> lpt(n);
+    for (int i = 0; i < n; i++) lpt[i] = i;
     auto steps_of = [&](int p) { return (double)(spec[p].n_run > 0 ? spec[p].n_run : spec[p].n_steps - spec[p].k0); };
-    std::stable_sort(B.order.begin(), B.order.end(), [&](int p, int q) {
+    std::stable_sort(lpt.begin(), lpt.end(), [&](int p, int q) {
         return (double)spec[p].geom->dev.nx * steps_of(p) > (double)spec[q].geom->dev.nx * steps_of(q);
     });
+    // one queue per XCD: an instance whose geometry reads shared lattice tables goes to the
+    // queue of its alignment (the 8 alignments of a lattice are 8 sets of lines; an XCD's 4 MB
+    // L2 then holds one of them), the others are dealt round-robin
+    static_assert(Lattice::kAlign == TDSE_QUEUES, "one queue per lattice alignment");
+    std::vector<int> per_queue[TDSE_QUEUES];
+    int dealt = 0;
+    for (int p : lpt) {
+        const int shift = spec[p].geom->lattice ? spec[p].geom->lattice_shift : -1;
+        per_queue[shift >= 0 ? shift : dealt++ % TDSE_QUEUES].push_back(p);
+    }
+    B.order.clear();
+    B.queues.off[0] = 0;
+    for (int q = 0; q < TDSE_QUEUES; q++) {
+        B.order.insert(B.order.end(), per_queue[q].begin(), per_queue[q].end());
+        B.queues.off[q + 1] = (int)B.order.size();
+    }
     B.gdev.resize(B.geoms.size());
     for (size_t k = 0; k < B.geoms.size(); k++) B.gdev[k] = B.geoms[k]->dev;
     return CTDSE_OK;
@@ -829,12 +849,12 @@ int grid_slots(int n, const LaunchShape &sh, int *slots)
 // Enqueues the queue reset and the persistent kernel (no allocation, copy or wait).
 int enqueue_plan(const ctdse_batch_plan &P, double2 *work, hipStream_t stream)
 {
-    hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, P.d_counter.as<int>(), 1, 0);
+    hipLaunchKernelGGL(tdse_fill_int_kernel, dim3(1), dim3(64), 0, stream, P.d_counter.as<int>(), TDSE_QUEUES, 0);
     const bool stash = P.shape.steps == 2 && P.shape.stash;
     hipLaunchKernelGGL(stash ? tdse_propagate_pair_stash_kernel
                              : (P.shape.steps == 2 ? tdse_propagate_pair_kernel : tdse_propagate_kernel),
                        dim3(P.slots), dim3(64 * P.shape.nw), stash ? stash_bytes(P.shape.nw) : 0, stream,
-                       P.d_geom.as<TdseGeom>(), P.d_inst.as<TdseInst>(), P.d_order.as<int>(), P.n,
+                       P.d_geom.as<TdseGeom>(), P.d_inst.as<TdseInst>(), P.d_order.as<int>(), P.queues,
                        P.d_counter.as<int>(), work, P.stride);
     HIP_CHECK(hipGetLastError());
     return CTDSE_OK;
@@ -849,6 +869,7 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream, const ctdse
     if (describe(spec, B) != CTDSE_OK) return g_status;
     ctdse_batch_plan P;
     P.n = n;
+    P.queues = B.queues;
     P.stride = B.stride;
     if (shape_for(B.geoms, cfg, &P.shape) != CTDSE_OK) return g_status;
     if (grid_slots(n, P.shape, &P.slots) != CTDSE_OK) return g_status;
@@ -862,7 +883,7 @@ int propagate(const std::vector<InstSpec> &spec, hipStream_t stream, const ctdse
     HIP_CHECK(P.d_geom.alloc(B.gdev.size() * sizeof(TdseGeom)));
     HIP_CHECK(P.d_inst.alloc(B.insts.size() * sizeof(TdseInst)));
     HIP_CHECK(P.d_order.alloc(B.order.size() * sizeof(int)));
-    HIP_CHECK(P.d_counter.alloc(sizeof(int)));
+    HIP_CHECK(P.d_counter.alloc(TDSE_QUEUES * sizeof(int)));
     HIP_CHECK(hipMemcpyAsync(P.d_geom.p, B.gdev.data(), B.gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(P.d_inst.p, B.insts.data(), B.insts.size() * sizeof(TdseInst), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(P.d_order.p, B.order.data(), B.order.size() * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -1020,6 +1041,7 @@ int make_plan(const std::vector<InstSpec> &spec, const ctdse_launch_config *cfg,
     BatchHost B;
     if (describe(spec, B) != CTDSE_OK) return g_status;
     P.n = n;
+    P.queues = B.queues;
     P.stride = B.stride;
     if (shape_for(B.geoms, cfg, &P.shape) != CTDSE_OK) return g_status;
     if (grid_slots(std::max(n, 1), P.shape, &P.slots) != CTDSE_OK) return g_status;
@@ -1027,7 +1049,7 @@ int make_plan(const std::vector<InstSpec> &spec, const ctdse_launch_config *cfg,
     HIP_CHECK(P.d_geom.alloc(B.gdev.size() * sizeof(TdseGeom)));
     HIP_CHECK(P.d_inst.alloc(B.insts.size() * sizeof(TdseInst)));
     HIP_CHECK(P.d_order.alloc(B.order.size() * sizeof(int)));
-    HIP_CHECK(P.d_counter.alloc(sizeof(int)));
+    HIP_CHECK(P.d_counter.alloc(TDSE_QUEUES * sizeof(int)));
     HIP_CHECK(hipMemcpy(P.d_geom.p, B.gdev.data(), B.gdev.size() * sizeof(TdseGeom), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(P.d_inst.p, B.insts.data(), B.insts.size() * sizeof(TdseInst), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(P.d_order.p, B.order.data(), B.order.size() * sizeof(int), hipMemcpyHostToDevice));

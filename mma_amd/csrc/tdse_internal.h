@@ -27,6 +27,15 @@
 #define TDSE_REV_WAVES_PER_SIMD 2  // reconstructing adjoint kernel: 256-VGPR budget, 8 waves per CU
 #define TDSE_MAX_HALO 40           // largest truncation depth in chunks
 
+// Instance queues of a persistent propagation launch, one per XCD: queue q holds
+// order[off[q] .. off[q + 1]).  A workgroup takes instances from the queue of the XCD it runs
+// on first, then from the others in turn, so each XCD's L2 mostly serves the tables of its own
+// queue's instances.
+#define TDSE_QUEUES 8
+struct TdseQueues {
+    int off[TDSE_QUEUES + 1];
+};
+
 // storage offset of chunk c (c >= -64), row i of the chunk
 __host__ __device__ inline long long tdse_chunk_base(long long c) { return ((c >> 6) + 1) * TDSE_TILE + (c & 63); }
 
@@ -156,12 +165,14 @@ __global__ void tdse_tables_ib_kernel(TableJob *jobs);
 // host forms of the serial setup steps (tdse_setup.hip), bit-identical to the kernels
 void tdse_ground_state_host(GsJob &J);
 void tdse_tables_pivot_host(TableJob &J);
-__global__ void tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
-                                      int *counter, double2 *work, long long work_stride);
-__global__ void tdse_propagate_pair_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order, int n_inst,
-                                           int *counter, double2 *work, long long work_stride);
+// counter: TDSE_QUEUES ints, zero at launch
+__global__ void tdse_propagate_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order,
+                                      TdseQueues queues, int *counter, double2 *work, long long work_stride);
+__global__ void tdse_propagate_pair_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order,
+                                           TdseQueues queues, int *counter, double2 *work, long long work_stride);
 __global__ void tdse_propagate_pair_stash_kernel(const TdseGeom *geoms, const TdseInst *insts, const int *order,
-                                                 int n_inst, int *counter, double2 *work, long long work_stride);
+                                                 TdseQueues queues, int *counter, double2 *work,
+                                                 long long work_stride);
 __global__ void tdse_untile_kernel(const double2 *psi, int nx, double *row);
 __global__ void tdse_parallel_step_kernel(const TdseGeom *geom, const TdseInst *insts, int k, const double2 *psi_in,
                                           double2 *psi_out, long long state_stride, double *partial, int nseg);

@@ -430,27 +430,53 @@ __device__ inline StepPhase step_phase(const TdseGeom &G, double F, int wave, in
 // LDS) and stored.  psi then streams through HBM once per two steps.
 // States saved for partial runs and snapshots are copied from the finished state between
 // steps (they are rare), so the segment solve carries no extra stores.
+// The XCD this wave runs on (hardware register XCC_ID, bits 3:0).
+__device__ inline int xcd_id()
+{
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return (int)(v & 15u);
+}
+
+// Next instance of the launch for this workgroup (thread 0 only): from the queue of this XCD
+// first, then the others in turn; tried (LDS) counts the queues found drained so far.  -1 when
+// every queue is drained: a queue is left only after its counter passed its length, so once
+// every workgroup has left every queue, each instance was taken exactly once.
+__device__ inline int next_instance(const TdseQueues &qs, int *counter, int &tried)
+{
+    const int q0 = xcd_id() % TDSE_QUEUES;
+    for (; tried < TDSE_QUEUES; tried++) {
+        const int q = (q0 + tried) % TDSE_QUEUES;
+        const int len = qs.off[q + 1] - qs.off[q];
+        if (len == 0) continue;
+        const int t = atomicAdd(counter + q, 1);
+        if (t < len) return qs.off[q] + t;
+    }
+    return -1;
+}
+
 template <bool PAIR, bool STASH = false>
 __device__ void propagate_loop(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
-                               const int *__restrict__ order, int n_inst, int *counter, double2 *work,
+                               const int *__restrict__ order, const TdseQueues &queues, int *counter, double2 *work,
                                long long work_stride, PairLds *pl, double2 *stash = nullptr)
 {
     __shared__ double part[2][TDSE_MAX_WAVES][5];
     __shared__ double2 slot_f[TDSE_MAX_WAVES][2], slot_b[TDSE_MAX_WAVES][2];
     __shared__ double2 edge[2][2], edge_b[2][2];
-    __shared__ int cur;
+    __shared__ int cur, tried;  // tried: thread 0's count of drained queues
     const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     double2 *const slot = work + (long long)blockIdx.x * work_stride;
     double2 R0[C], R1[C];  // pair passes: step-k+1 state of the last two segments solved
+    if (threadIdx.x == 0) tried = 0;
 
     for (;;) {
-        if (threadIdx.x == 0) cur = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) cur = next_instance(queues, counter, tried);
         __syncthreads();
         const int t = __builtin_amdgcn_readfirstlane(cur);  // uniform: keeps the instance in SGPRs
         __syncthreads();
-        if (t >= n_inst) break;
+        if (t < 0) break;
         const TdseInst &I = insts[order[t]];
         const TdseGeom &G = geoms[I.geom];
         const long long npad = G.npad;
@@ -624,31 +650,31 @@ __device__ inline void store_partials(const TdseInst &I, const double *parts, in
 // One time step per pass over psi (168-VGPR budget, 12 waves per CU).
 __global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_WAVES_PER_SIMD)
 tdse_propagate_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
-                      const int *__restrict__ order, int n_inst, int *counter, double2 *work, long long work_stride)
+                      const int *__restrict__ order, TdseQueues queues, int *counter, double2 *work, long long work_stride)
 {
-    propagate_loop<false>(geoms, insts, order, n_inst, counter, work, work_stride, nullptr);
+    propagate_loop<false>(geoms, insts, order, queues, counter, work, work_stride, nullptr);
 }
 
 // Two time steps per pass wherever no save or snapshot falls between them.
 __global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_PAIR_WAVES_PER_SIMD)
 tdse_propagate_pair_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
-                           const int *__restrict__ order, int n_inst, int *counter, double2 *work,
+                           const int *__restrict__ order, TdseQueues queues, int *counter, double2 *work,
                            long long work_stride)
 {
     __shared__ PairLds pl;
-    propagate_loop<true>(geoms, insts, order, n_inst, counter, work, work_stride, &pl);
+    propagate_loop<true>(geoms, insts, order, queues, counter, work, work_stride, &pl);
 }
 
 // The same, with each segment's 1/beta kept in LDS between the two stages of a pass (dynamic
 // LDS of 2 * 64 * waves * 8 complex doubles): the second stage does not fetch it again.
 __global__ void __launch_bounds__(64 * TDSE_MAX_WAVES, TDSE_PAIR_WAVES_PER_SIMD)
 tdse_propagate_pair_stash_kernel(const TdseGeom *__restrict__ geoms, const TdseInst *__restrict__ insts,
-                                 const int *__restrict__ order, int n_inst, int *counter, double2 *work,
+                                 const int *__restrict__ order, TdseQueues queues, int *counter, double2 *work,
                                  long long work_stride)
 {
     __shared__ PairLds pl;
     extern __shared__ double2 stash[];
-    propagate_loop<true, true>(geoms, insts, order, n_inst, counter, work, work_stride, &pl, stash);
+    propagate_loop<true, true>(geoms, insts, order, queues, counter, work, work_stride, &pl, stash);
 }
 
 // One time step k of instance blockIdx.y, split over independent segments (blockIdx.x) whose
