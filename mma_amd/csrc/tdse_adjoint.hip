@@ -51,8 +51,8 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
     }
     if (gl == own_chunks - 1) edge_out[0] = P[C - 1];
     const double2 zero2 = make_double2(0.0, 0.0);
-    double2 left = dpp2<kWaveShr1>(P[C - 1], zero2), right = dpp2<kWaveShl1>(P[0], zero2);
-    double2 ibL = dpp2<kWaveShr1>(IB[C - 1], zero2);
+    double2 left = dpp2_z<kWaveShr1>(P[C - 1]), right = dpp2_z<kWaveShl1>(P[0]);
+    double2 ibL = dpp2_z<kWaveShr1>(IB[C - 1]);
     if (lane == 0) {
         const long long nb = tdse_chunk_base(min(c - 1, G.nchunks)) + 64 * (C - 1);  // chunk -1: zero padding
         left = gl > 0 ? ld2(kap, nb) : (seg > 0 ? edge_in[0] : zero2);  // gl > 0: not yet overwritten

@@ -237,11 +237,11 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     // neighbours across the chunk edges: shuffles inside a wave, loads across waves, LDS for
     // the point left of the window, zero past the window's right end
     {
-        const double gLd = dpp<kWaveShr1>(Gv[C - 1], 0.0), gRd = dpp<kWaveShl1>(Gv[0], 0.0);
+        const double gLd = dpp_z<kWaveShr1>(Gv[C - 1]), gRd = dpp_z<kWaveShl1>(Gv[0]);
         gL = lane == 0 ? gL : gLd;
         gR = lane == 63 ? gR : gRd;
     }
-    double2 left = dpp2<kWaveShr1>(P[C - 1], zero2), right = dpp2<kWaveShl1>(P[0], zero2);
+    double2 left = dpp2_z<kWaveShr1>(P[C - 1]), right = dpp2_z<kWaveShl1>(P[0]);
     if (lane == 0) left = (gl > 0 || MODE == kParallel) ? nb : (seg > 0 ? edge_in[0] : zero2);
     if (lane == 63) right = nb;
     // the forward recurrence enters exactly from the previous segment, or from zero H chunks

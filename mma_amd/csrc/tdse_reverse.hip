@@ -84,7 +84,7 @@ __device__ void reverse_segment(const TdseGeom &G, int seg, int own_chunks, int 
     const double2 Qa = ld2(G.adq, cl + 64), Ga = ld2(G.adg, cl + 64);
     const double2 Qf = ld2(G.chq, cl + 64), Gf = ld2(G.chg, cl + 64);
     {
-        const double2 ibd = dpp2<kWaveShr1>(IB[C - 1], zero2);  // 1/beta left of the chunk, inside the wave
+        const double2 ibd = dpp2_z<kWaveShr1>(IB[C - 1]);  // 1/beta left of the chunk, inside the wave
         ibL = lane == 0 ? ibL : ibd;
     }
     const bool own = gl < own_chunks && c < G.nchunks;
@@ -94,7 +94,7 @@ __device__ void reverse_segment(const TdseGeom &G, int seg, int own_chunks, int 
     // phi = conj(e psi) (into S)
     double2 e_end;  // e at the point right of the chunk
     {
-        double2 pm = dpp2<kWaveShr1>(Kp[C - 1], zero2), kright = dpp2<kWaveShl1>(Kp[0], zero2);
+        double2 pm = dpp2_z<kWaveShr1>(Kp[C - 1]), kright = dpp2_z<kWaveShl1>(Kp[0]);
         if (lane == 0) pm = gl > 0 ? kl : (seg > 0 ? edge_in[0] : zero2);
         if (lane == 63) kright = kr;
         const int j0 = c * C;
@@ -127,7 +127,7 @@ __device__ void reverse_segment(const TdseGeom &G, int seg, int own_chunks, int 
     }
     if (gl == own_chunks - 1) edge_out[2] = S[C - 1];
     // phi across the chunk edges for the psi solve; phi itself waits in LDS
-    double2 left = dpp2<kWaveShr1>(S[C - 1], zero2), right = dpp2<kWaveShl1>(S[0], zero2);
+    double2 left = dpp2_z<kWaveShr1>(S[C - 1]), right = dpp2_z<kWaveShl1>(S[0]);
     if (lane == 0) left = gl > 0 ? conj2(cmul(cmul(e0, conj2(rot_conj)), sl)) : (seg > 0 ? edge_in[2] : zero2);
     if (lane == 63) right = conj2(cmul(e_end, sr));
     if (step_psi) {
@@ -192,7 +192,7 @@ __device__ void reverse_segment(const TdseGeom &G, int seg, int own_chunks, int 
 #pragma unroll
     for (int i = 0; i < C; i++) S[i] = park[64 * i + lane];  // this lane's own writes
     {
-        const double gLd = dpp<kWaveShr1>(Gv[C - 1], 0.0), gRd = dpp<kWaveShl1>(Gv[0], 0.0);
+        const double gLd = dpp_z<kWaveShr1>(Gv[C - 1]), gRd = dpp_z<kWaveShl1>(Gv[0]);
         gL = lane == 0 ? gL : gLd;
         gR = lane == 63 ? gR : gRd;
     }

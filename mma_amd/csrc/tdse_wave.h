@@ -67,6 +67,30 @@ __device__ inline double2 dpp2(double2 v, double2 old)
 {
     return make_double2(dpp<CTRL, ROWS>(v.x, old.x), dpp<CTRL, ROWS>(v.y, old.y));
 }
+// Full-row-mask moves whose lanes without a source take 0 (bound_ctrl): no `old` to set up
+// first.  dpp2_one: a complex value whose lanes without a source take 1 + 0i (only the high
+// half of the real part needs an `old`).
+template <int CTRL>
+__device__ inline double dpp_z(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL>
+__device__ inline double2 dpp2_z(double2 v)
+{
+    return make_double2(dpp_z<CTRL>(v.x), dpp_z<CTRL>(v.y));
+}
+template <int CTRL>
+__device__ inline double2 dpp2_one(double2 v)
+{
+    const long long b = __builtin_bit_cast(long long, v.x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0x3ff00000, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return make_double2(__builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo), dpp_z<CTRL>(v.y));
+}
 __device__ inline double readlane(double v, int l)
 {
     const long long b = __builtin_bit_cast(long long, v);
@@ -93,8 +117,8 @@ __device__ inline void wave_scan(double2 &Q, double2 &z, int lane)
     const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
 #define TDSE_SCAN_LEVEL(D)                                                                                   \
     {                                                                                                        \
-        const double2 Qo = FWD ? dpp2<kRowShr + D>(Q, one) : dpp2<kRowShl + D>(Q, one);                      \
-        const double2 zo = FWD ? dpp2<kRowShr + D>(z, zero) : dpp2<kRowShl + D>(z, zero);                    \
+        const double2 Qo = FWD ? dpp2_one<kRowShr + D>(Q) : dpp2_one<kRowShl + D>(Q);                        \
+        const double2 zo = FWD ? dpp2_z<kRowShr + D>(z) : dpp2_z<kRowShl + D>(z);                            \
         compose(Q, z, Qo, zo);                                                                               \
     }
     TDSE_SCAN_LEVEL(1)
@@ -159,8 +183,8 @@ __device__ inline double2 block_carry(double2 Qi, double2 zi, int wave, int lane
 {
     // exclusive within the wave: shift the inclusive maps by one lane
     const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-    const double2 Qe = FWD ? dpp2<kWaveShr1>(Qi, one) : dpp2<kWaveShl1>(Qi, one);
-    const double2 ze = FWD ? dpp2<kWaveShr1>(zi, zero) : dpp2<kWaveShl1>(zi, zero);
+    const double2 Qe = FWD ? dpp2_one<kWaveShr1>(Qi) : dpp2_one<kWaveShl1>(Qi);
+    const double2 ze = FWD ? dpp2_z<kWaveShr1>(zi) : dpp2_z<kWaveShl1>(zi);
     if (lane == (FWD ? 63 : 0)) {
         slot[wave][0] = Qi;
         slot[wave][1] = zi;
