@@ -249,25 +249,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     const double2 z_in = (MODE != kParallel && seg > 0) ? edge_in[1] : zero2;
 
     // r' = (M2' - i H') psi
-    {
-        double2 pm = left;
-        double gm = gL;
-        const double K = G.k36;
-#pragma unroll
-        for (int i = 0; i < C; i++) {
-            const double2 p0 = P[i];
-            const double2 pp = (i + 1 < C) ? P[i + 1] : right;
-            const double gp = (i + 1 < C) ? Gv[i + 1] : gR;
-            const double d = fma(10.0, Gv[i], K);
-            const double mr = fma(10.0, p0.x, pm.x + pp.x);
-            const double mi = fma(10.0, p0.y, pm.y + pp.y);
-            const double hr = fma(gm, pm.x, fma(d, p0.x, gp * pp.x));
-            const double hi = fma(gm, pm.y, fma(d, p0.y, gp * pp.y));
-            P[i] = make_double2(mr + hi, mi - hr);
-            pm = p0;
-            gm = Gv[i];
-        }
-    }
+    cn_rhs(P, Gv, left, right, gL, gR, G.k36);
 
     // forward recurrence z_j = ib_j (r_j - (1 + i g_{j-1}) z_{j-1})
     {

@@ -196,25 +196,7 @@ __device__ void reverse_segment(const TdseGeom &G, int seg, int own_chunks, int 
         gL = lane == 0 ? gL : gLd;
         gR = lane == 63 ? gR : gRd;
     }
-    {
-        double2 pm = left;
-        double gm = gL;
-        const double K36 = G.k36;
-#pragma unroll
-        for (int i = 0; i < C; i++) {
-            const double2 p0 = S[i];
-            const double2 pp = (i + 1 < C) ? S[i + 1] : right;
-            const double gp = (i + 1 < C) ? Gv[i + 1] : gR;
-            const double d = fma(10.0, Gv[i], K36);
-            const double mr = fma(10.0, p0.x, pm.x + pp.x);
-            const double mi = fma(10.0, p0.y, pm.y + pp.y);
-            const double hr = fma(gm, pm.x, fma(d, p0.x, gp * pp.x));
-            const double hi = fma(gm, pm.y, fma(d, p0.y, gp * pp.y));
-            S[i] = make_double2(mr + hi, mi - hr);
-            pm = p0;
-            gm = Gv[i];
-        }
-    }
+    cn_rhs(S, Gv, left, right, gL, gR, G.k36);
     {
         double2 z = zero2;
 #pragma unroll

@@ -160,6 +160,32 @@ __device__ inline void wave_scan(double2 &Q, double2 &z, int lane)
     compose(Q, z, CQ, CZ);
 }
 
+// r' = (M2' - i H') psi over one lane's chunk P (in place), M2' = (1, 10, 1) and
+// H' = (g_{j-1}, 10 g_j + K, g_{j+1}); left / right: the points either side of the chunk, gL /
+// gR their g.  Real part summed as pm.x + gm pm.y + pp.x + gp pp.y + 10 p0.x + d p0.y (the
+// imaginary part likewise): 12 instructions per point.
+__device__ inline void cn_rhs(double2 (&P)[C], const double (&Gv)[C], double2 left, double2 right, double gL,
+                              double gR, double K)
+{
+    const double ten = 10.0;
+    double2 pm = left;
+    double gm = gL;
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const double2 p0 = P[i];
+        const double2 pp = (i + 1 < C) ? P[i + 1] : right;
+        const double gp = (i + 1 < C) ? Gv[i + 1] : gR;
+        const double d = fma(ten, Gv[i], K);
+        double re = fma(gp, pp.y, fma(gm, pm.y, pm.x) + pp.x);
+        double im = fma(-gp, pp.x, fma(-gm, pm.x, pm.y) + pp.y);
+        re = fma(d, p0.y, fma(ten, p0.x, re));
+        im = fma(-d, p0.x, fma(ten, p0.y, im));
+        P[i] = make_double2(re, im);
+        pm = p0;
+        gm = Gv[i];
+    }
+}
+
 struct Acc {
     double pr, pi, pint, pos, gp;
 };
