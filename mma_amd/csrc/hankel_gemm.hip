@@ -265,7 +265,9 @@ __device__ inline void split_gemm(int M, int N, int K, const unsigned short *A, 
         } else {
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
-        __builtin_amdgcn_s_barrier();  // every wave's copies of stage s are in LDS
+        // every wave's copies of stage s are in LDS (a barrier with a memory clobber: no LDS
+        // access moves across it)
+        asm volatile("s_barrier" ::: "memory");
 #pragma unroll
         for (int kk = 0; kk < S::SBK; kk += 16) {
             const int c = kk / 8 + kh;
@@ -314,7 +316,7 @@ __device__ inline void split_gemm(int M, int N, int K, const unsigned short *A, 
                     }
                 }
         }
-        __builtin_amdgcn_s_barrier();  // buffer `buf` is free for stage s + 2
+        asm volatile("s_barrier" ::: "memory");  // buffer `buf` is free for stage s + 2
     }
 #pragma unroll
     for (int tj = 0; tj < 2; tj++) {
