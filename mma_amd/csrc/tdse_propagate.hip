@@ -95,6 +95,25 @@ __device__ inline void load_tables(const TdseGeom &G, int cs, int wave, int lane
         // past this grid, and the point left of it, read as zero; the first H chunks and the
         // last one keep their own pivots and chunk maps (the walls of this grid)
         const long long sb = tdse_chunk_base(cl + G.s_off);
+        const int w0 = cs + wave * 64;  // this wave's first chunk
+        if (w0 > 0 && w0 >= G.head_chunks && w0 + 63 < G.nchunks - 1) {
+            // (uniform) a wave with no head chunk and none of the last two chunks: every row and
+            // the right neighbour on the grid, lattice rows and maps only, nothing to mask
+            T.Qc = ld2(G.schq, cl + G.s_off + 64);
+            T.Gc = ld2(G.schg, cl + G.s_off + 64);
+#pragma unroll
+            for (int i = 0; i < C; i++) {
+                T.Gv[i] = ld1(G.sg, sb + 64 * i);
+                T.dp[i] = ld1(G.sdp, sb + 64 * i);
+                if constexpr (IB_LDS)
+                    T.IB[i] = stash[i * 64 * nw + gl];
+                else
+                    T.IB[i] = ld2(G.sib, sb + 64 * i);
+            }
+            T.gL = lane == 0 ? ld1(G.sg, tdse_chunk_base(c - 1 + G.s_off) + 64 * (C - 1)) : 0.0;
+            T.gR = lane == 63 && gl < nwl * 64 - 1 ? ld1(G.sg, tdse_chunk_base(c + 1 + G.s_off)) : 0.0;
+            return;
+        }
         const bool head = c < G.head_chunks;
         const bool own_maps = head || c >= G.nchunks - 1;
         T.Qc = ld2(own_maps ? G.chq : G.schq, (own_maps ? cl : cl + G.s_off) + 64);
