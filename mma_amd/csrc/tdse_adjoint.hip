@@ -86,7 +86,7 @@ __device__ void adjoint_segment(const TdseGeom &G, int seg, int own_chunks, int 
             const double w = gobs * ld1(G.dpot, base + 64 * i);
             lr = fma(-w, ps.x, lr);
             li = fma(-w, ps.y, li);
-            const double xj = __dsub_rn(__dmul_rn((double)(j0 + i), G.dx), G.xmax);
+            const double xj = grid_x(j0 + i, G.dx, G.xmax);
             gsum = fma(xj, fma(lr, ps.y, -li * ps.x), gsum);
             P[i] = cmul(e, make_double2(lr, li));
             e = cmul(e, rot_conj);
@@ -184,7 +184,7 @@ tdse_adjoint_kernel(const TdseGeom *__restrict__ geoms, const TdseAdjInst *__res
             // from one segment to the next
             double2 e_lane, r_seg;
             {
-                const double x = __dsub_rn(__dmul_rn((double)((wave * 64 + lane) * C), G.dx), G.xmax);
+                const double x = grid_x((wave * 64 + lane) * C, G.dx, G.xmax);
                 sincos(-Fdt * x, &e_lane.y, &e_lane.x);
                 sincos(-Fdt * ((double)(own_chunks * C) * G.dx), &r_seg.y, &r_seg.x);
             }

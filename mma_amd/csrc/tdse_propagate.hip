@@ -55,7 +55,7 @@ __device__ inline void observe_chunk(const TdseGeom &G, const double2 (&P)[C], c
     for (int i = 0; i < C; i++) {
         const double2 v = P[i];
         const double n2 = fma(v.x, v.x, v.y * v.y);
-        const double xj = __dsub_rn(__dmul_rn((double)(j0 + i), G.dx), G.xmax);
+        const double xj = grid_x(j0 + i, G.dx, G.xmax);
         acc.pos = fma(n2, xj, acc.pos);
         acc.gp = fma(n2, dp[i], acc.gp);
         const int j = j0 + i;
@@ -418,7 +418,7 @@ __device__ inline StepPhase step_phase(const TdseGeom &G, double F, int wave, in
     StepPhase p;
     const double Fdt = F * G.dt;
     sincos(Fdt * G.dx, &p.rot.y, &p.rot.x);
-    const double x = __dsub_rn(__dmul_rn((double)((wave * 64 + lane) * C), G.dx), G.xmax);
+    const double x = grid_x((wave * 64 + lane) * C, G.dx, G.xmax);
     sincos(Fdt * x, &p.e_lane.y, &p.e_lane.x);
     sincos(Fdt * ((double)(own_chunks * C) * G.dx), &p.r_seg.y, &p.r_seg.x);
     return p;
@@ -714,7 +714,7 @@ tdse_parallel_step_kernel(const TdseGeom *__restrict__ geom, const TdseInst *__r
     double2 rot, e0;
     sincos(Fdt * G.dx, &rot.y, &rot.x);
     const int c = seg * own_chunks - G.halo_lanes + wave * 64 + lane;  // this lane's chunk
-    sincos(Fdt * __dsub_rn(__dmul_rn((double)(c * C), G.dx), G.xmax), &e0.y, &e0.x);
+    sincos(Fdt * grid_x(c * C, G.dx, G.xmax), &e0.y, &e0.x);
     Acc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
     double2 R[C];  // unused by this mode
     solve_segment<kParallel>(G, seg, own_chunks, wave, lane, nw,

@@ -115,7 +115,7 @@ __device__ void reverse_segment(const TdseGeom &G, int seg, int own_chunks, int 
             const double w = gobs * dp[i];
             lr = fma(-w, ps.x, lr);
             li = fma(-w, ps.y, li);
-            const double xj = __dsub_rn(__dmul_rn((double)(j0 + i), G.dx), G.xmax);
+            const double xj = grid_x(j0 + i, G.dx, G.xmax);
             gsum = fma(xj, fma(lr, ps.y, -li * ps.x), gsum);
             Kp[i] = cmul(e, make_double2(lr, li));
             S[i] = conj2(cmul(e, ps));
@@ -295,7 +295,7 @@ tdse_reverse_adjoint_kernel(const TdseGeom *__restrict__ geoms, const TdseRevIns
             const double2 rot_conj = make_double2(rc, rs);
             double2 e_lane, r_seg;  // e at this lane's first chunk in segment 0; segment-to-segment rotation
             {
-                const double x = __dsub_rn(__dmul_rn((double)((wave * 64 + lane) * C), G.dx), G.xmax);
+                const double x = grid_x((wave * 64 + lane) * C, G.dx, G.xmax);
                 sincos(-Fdt * x, &e_lane.y, &e_lane.x);
                 sincos(-Fdt * ((double)(own_chunks * C) * G.dx), &r_seg.y, &r_seg.x);
             }

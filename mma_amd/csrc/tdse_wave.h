@@ -42,6 +42,10 @@ __device__ inline double ld1(const double *p, long long i) { return ((const TDSE
 // __syncthreads() in between.
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// x_j = j dx - xmax in one rounding (an FMA), within an ulp of the grid's own x_j (a product
+// and a difference, rounded twice)
+__device__ inline double grid_x(int j, double dx, double xmax) { return fma((double)j, dx, -xmax); }
+
 __device__ inline double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
