@@ -39,7 +39,7 @@ BYTES_PER_PT_STEP = 32.0  # psi read + write, complex128
 # HBM bytes per launch of tdse_propagate_kernel on the default 64 x 64 workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE counters scaled by their readings for a plain 1 GiB device copy
 # (x2.0 and x1.0); commands and readings in profiles/tdse_propagate_gaussian_beam_4096.md
-MEASURED_TRAFFIC_BYTES = 5.115e12 + 4.684e12
+MEASURED_TRAFFIC_BYTES = 5.105e12 + 4.685e12
 
 
 def log(*a):
