@@ -326,21 +326,33 @@ def hankel_measure(cpu=True):
     w[1:] += 0.5 * dz
     Wt = np.stack([w, np.eye(nz)[0], np.eye(nz)[-1]])
 
+    tabs = []
+
     def run(kind, sources=src):
-        tab = H._PreFactor(zg, rg, og, "Ar", pressure, "Henke", True, "NIST", True, 1.0)
+        # the pre-factor tables are queued with the contraction; their table-error flag is read
+        # after the pass (check_tables), as Hankel_long does before it hands results over
+        tab = H._PreFactor(zg, rg, og, "Ar", pressure, "Henke", True, "NIST", True, 1.0, defer=True)
+        tabs.append(tab)
         if kind == "fp64":
             return H.direct(sources, zg, og, rg, rho, distance, weights=Wt, pre=tab.pre)
         return H.gridded(sources, zg, og, rg, rho, distance, Wt, pre=tab.pre, kind=kind)
+
+    def check_tables():
+        for t in tabs:
+            t.check()
+        tabs.clear()
 
     out, res = {}, {}
     for kind in ("fp64", "fp64-grid", "fp32-mfma", "bf16x3-mfma", "bf16x6-mfma", "fp16x3-mfma"):
         run(kind)  # warm-up (code objects, allocations)
         torch.cuda.synchronize()
         reps = 1 if kind == "fp64" else 5
+        check_tables()
         t0 = time.perf_counter()
         for _ in range(reps):
             r = run(kind)
         torch.cuda.synchronize()
+        check_tables()
         out[kind] = (time.perf_counter() - t0) / reps
         res[kind] = r
         log("hankel %s: %.2f ms" % (kind, out[kind] * 1e3))
@@ -350,10 +362,12 @@ def hankel_measure(cpu=True):
     for kind in ("fp32-mfma", "bf16x3-mfma", "bf16x6-mfma", "fp16x3-mfma"):
         run(kind, src64)
         torch.cuda.synchronize()
+        check_tables()
         t0 = time.perf_counter()
         for _ in range(5):
             r = run(kind, src64)
         torch.cuda.synchronize()
+        check_tables()
         out64[kind] = (time.perf_counter() - t0) / 5
         res64[kind] = r
         log("hankel %s, complex64 sources: %.2f ms" % (kind, out64[kind] * 1e3))

@@ -64,6 +64,19 @@ typedef struct {
 int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const double *ogrid, int nr,
                  const double *rgrid, double *pre, double *renorm, void *stream);
 
+/* The same tables, queued on `stream` with nothing waiting (hk_prefactor waits for the tables to
+ * report a table error at once).  flag: device int, cleared here and raised by the kernels when
+ * a photon energy lies outside the scattering-factor tables or has no f1 listed; once the
+ * stream has passed this work the caller reads it and hands it to hk_prefactor_status, which
+ * returns what hk_prefactor would have (HK_OK or HK_ERR_TABLE: the reference's interpolation
+ * out of range, or its XUV_refractive_index.py:156 error).  work: `no` doubles of device scratch for a z-tabulated
+ * pressure (else unused, may be NULL).  Calls that share flag or work must be ordered on one
+ * stream.  No counterpart in the reference (its pre-factor function raises at once); for
+ * callers that keep a far-field pass free of host waits. */
+int hk_prefactor_enqueue(const hk_medium *m, int nz, const double *zgrid, int no, const double *ogrid, int nr,
+                         const double *rgrid, double *pre, double *renorm, int *flag, double *work, void *stream);
+int hk_prefactor_status(int flag);
+
 /* ---- the contraction over r and z ------------------------------------------------------- */
 
 /* Source planes: src + k * plane_stride is plane k, laid out [o][j] (no x nr, row-major) as

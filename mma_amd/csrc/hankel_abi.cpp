@@ -133,15 +133,12 @@ bool uniform_radii(const std::vector<double> &r, double *dr_out)
 
 } // namespace
 
-extern "C" {
+namespace {
 
-int hk_last_error(void) { return g_status; }
-const char *hk_last_error_message(void) { return g_message.c_str(); }
-
-int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const double *ogrid, int nr,
-                 const double *rgrid, double *pre, double *renorm, void *stream)
+// The argument checks shared by both pre-factor entry points.
+int check_prefactor_args(const hk_medium *m, int nz, const double *zgrid, int no, const double *ogrid, int nr,
+                         const double *rgrid, double *pre, double *renorm)
 {
-    clear_status();
     if (!m || nz < 1 || no < 1 || !zgrid || !ogrid || !pre) return fail(HK_ERR_ARGUMENT, "bad pre-factor arguments");
     if (m->mode < HK_PRESSURE_SCALAR || m->mode > HK_PRESSURE_ZR) return fail(HK_ERR_ARGUMENT, "unknown pressure mode");
     if (m->mode != HK_PRESSURE_ZR && !renorm)
@@ -153,6 +150,50 @@ int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const 
     if ((m->include_dispersion && (m->n_f1 < 1 || !m->e_f1 || !m->f1)) ||
         (m->include_absorption && (m->n_f2 < 1 || !m->e_f2 || !m->f2)))
         return fail(HK_ERR_ARGUMENT, "missing scattering-factor table");
+    return HK_OK;
+}
+
+// Queues the pre-factor kernels after clearing *flag (a fill kernel, no memset node): nothing
+// waits.  exit_abs: no doubles of device scratch (z-tabulated pressure).
+hipError_t enqueue_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const double *ogrid, int nr,
+                             const double *rgrid, double *pre, double *renorm, int *flag, double *exit_abs,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(hk_fill_int_kernel, dim3(1), dim3(64), 0, s, flag, 1, 0);
+    const int n = m->mode == HK_PRESSURE_ZR ? no * nr : no;
+    // the scalar branch: one thread per (frequency, plane)
+    const unsigned planes = m->mode == HK_PRESSURE_SCALAR ? (unsigned)std::min(nz, 65535) : 1u;
+    // the z-tabulated walk reads the plane grid and the pressure table from LDS when they fit
+    const size_t stage = (size_t)(nz + 2 * m->np_z) * sizeof(double);
+    const int staged = m->mode == HK_PRESSURE_Z && stage <= 48 * 1024;
+    if (m->mode == HK_PRESSURE_Z && m->np_z <= kWalkParallelMax && nz <= 65535)
+        hipLaunchKernelGGL(hk_prefactor_z_kernel, dim3((no + 127) / 128, nz), dim3(128), 0, s, make_constants(), *m,
+                           nz, zgrid, no, ogrid, reinterpret_cast<double2 *>(pre), renorm, exit_abs, flag);
+    else
+        hipLaunchKernelGGL(hk_prefactor_kernel, dim3((n + 127) / 128, planes), dim3(128), staged ? stage : 0, s,
+                           make_constants(), *m, nz, zgrid, no, ogrid, nr, rgrid, reinterpret_cast<double2 *>(pre),
+                           renorm, exit_abs, flag, staged);
+    if (m->mode == HK_PRESSURE_Z) {
+        const long long total = (long long)nz * no;
+        hipLaunchKernelGGL(hk_prefactor_finish_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nz, no,
+                           ogrid, exit_abs, reinterpret_cast<double2 *>(pre), renorm);
+    }
+    return hipGetLastError();
+}
+
+} // namespace
+
+extern "C" {
+
+int hk_last_error(void) { return g_status; }
+const char *hk_last_error_message(void) { return g_message.c_str(); }
+
+
+int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const double *ogrid, int nr,
+                 const double *rgrid, double *pre, double *renorm, void *stream)
+{
+    clear_status();
+    if (check_prefactor_args(m, nz, zgrid, no, ogrid, nr, rgrid, pre, renorm) != HK_OK) return g_status;
     hipStream_t s = (hipStream_t)stream;
     // scratch kept between calls (an allocation and a free per call would each synchronise the
     // device): the failure flag and, per frequency, the z-tabulated absorption at the exit plane
@@ -176,34 +217,32 @@ int hk_prefactor(const hk_medium *m, int nz, const double *zgrid, int no, const 
         scratch_dev = dev;
         exit_n = no;
     }
-    int zero = 0, got = 0;
-    hipError_t e = hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, s);
-    const int n = m->mode == HK_PRESSURE_ZR ? no * nr : no;
-    if (e == hipSuccess) {
-        // the scalar branch: one thread per (frequency, plane)
-        const unsigned planes = m->mode == HK_PRESSURE_SCALAR ? (unsigned)std::min(nz, 65535) : 1u;
-        // the z-tabulated walk reads the plane grid and the pressure table from LDS when they fit
-        const size_t stage = (size_t)(nz + 2 * m->np_z) * sizeof(double);
-        const int staged = m->mode == HK_PRESSURE_Z && stage <= 48 * 1024;
-        if (m->mode == HK_PRESSURE_Z && m->np_z <= kWalkParallelMax && nz <= 65535)
-            hipLaunchKernelGGL(hk_prefactor_z_kernel, dim3((no + 127) / 128, nz), dim3(128), 0, s, make_constants(), *m,
-                               nz, zgrid, no, ogrid, reinterpret_cast<double2 *>(pre), renorm, exit_abs, flag);
-        else
-            hipLaunchKernelGGL(hk_prefactor_kernel, dim3((n + 127) / 128, planes), dim3(128), staged ? stage : 0, s,
-                               make_constants(), *m, nz, zgrid, no, ogrid, nr, rgrid, reinterpret_cast<double2 *>(pre),
-                               renorm, exit_abs, flag, staged);
-        if (m->mode == HK_PRESSURE_Z) {
-            const long long total = (long long)nz * no;
-            hipLaunchKernelGGL(hk_prefactor_finish_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nz,
-                               no, ogrid, exit_abs, reinterpret_cast<double2 *>(pre), renorm);
-        }
-        e = hipGetLastError();
-    }
+    int got = 0;
+    hipError_t e = enqueue_prefactor(m, nz, zgrid, no, ogrid, nr, rgrid, pre, renorm, flag, exit_abs, s);
     if (e == hipSuccess) e = hipMemcpyAsync(&got, flag, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail(HK_ERR_DEVICE, std::string("pre-factor kernel: ") + hipGetErrorString(e));
-    if (got == HK_FLAG_RANGE) return fail(HK_ERR_TABLE, "photon energy outside the tabulated scattering factors");
-    if (got == HK_FLAG_F1) return fail(HK_ERR_TABLE, "f1-scattering coefficient not listed for this frequency.");
+    return hk_prefactor_status(got);
+}
+
+int hk_prefactor_enqueue(const hk_medium *m, int nz, const double *zgrid, int no, const double *ogrid, int nr,
+                         const double *rgrid, double *pre, double *renorm, int *flag, double *work, void *stream)
+{
+    clear_status();
+    if (check_prefactor_args(m, nz, zgrid, no, ogrid, nr, rgrid, pre, renorm) != HK_OK) return g_status;
+    if (!flag || (m->mode == HK_PRESSURE_Z && !work))
+        return fail(HK_ERR_ARGUMENT, "the deferred pre-factor needs a flag and, for a z-tabulated pressure, work");
+    const hipError_t e = enqueue_prefactor(m, nz, zgrid, no, ogrid, nr, rgrid, pre, renorm, flag, work,
+                                           (hipStream_t)stream);
+    if (e != hipSuccess) return fail(HK_ERR_DEVICE, std::string("pre-factor kernel: ") + hipGetErrorString(e));
+    return HK_OK;
+}
+
+int hk_prefactor_status(int flag)
+{
+    clear_status();
+    if (flag == HK_FLAG_RANGE) return fail(HK_ERR_TABLE, "photon energy outside the tabulated scattering factors");
+    if (flag == HK_FLAG_F1) return fail(HK_ERR_TABLE, "f1-scattering coefficient not listed for this frequency.");
     return HK_OK;
 }
 

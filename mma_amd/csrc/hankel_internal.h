@@ -20,6 +20,8 @@ enum HkErrFlag { HK_FLAG_OK = 0, HK_FLAG_RANGE = 1, HK_FLAG_F1 = 2 };
 __global__ void hk_prefactor_kernel(HkConst K, hk_medium m, int nz, const double *zgrid, int no, const double *ogrid,
                                     int nr, const double *rgrid, double2 *pre, double *renorm, double *exit_abs,
                                     int *flag, int staged);
+// p[0 .. n) = v (flags and counters cleared by a kernel, not a memset node)
+__global__ void hk_fill_int_kernel(int *p, int n, int v);
 __global__ void hk_prefactor_z_kernel(HkConst K, hk_medium m, int nz, const double *zgrid, int no, const double *ogrid,
                                       double2 *pre, double *renorm, double *exit_abs, int *flag);
 constexpr int kWalkParallelMax = 4096;  // pressure tables up to this length: one thread per (frequency, plane)

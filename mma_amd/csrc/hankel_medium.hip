@@ -318,3 +318,9 @@ __global__ void hk_prefactor_finish_kernel(int nz, int no, const double *ogrid, 
     pre[at] = make_double2(pm * (e * cs), pm * (e * s));
     renorm[at] = exp(w * last_abs - w * v.x);
 }
+
+__global__ void hk_fill_int_kernel(int *p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}

@@ -68,6 +68,9 @@ def _L():
         L.hk_last_error_message.restype = ctypes.c_char_p
         L.hk_prefactor.argtypes = [POINTER(hk_medium), c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
                                    c_void_p, c_void_p]
+        L.hk_prefactor_enqueue.argtypes = [POINTER(hk_medium), c_int, c_void_p, c_int, c_void_p, c_int, c_void_p,
+                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+        L.hk_prefactor_status.argtypes = [c_int]
         L.hk_direct.argtypes = [POINTER(hk_planes), c_int, c_void_p, c_int, c_void_p, c_void_p]
         L.hk_grid_size.argtypes = [c_double, c_double, c_double, POINTER(c_int), POINTER(c_double)]
         L.hk_grid_spread.argtypes = [POINTER(hk_planes), c_int, c_void_p, c_int, c_double, c_int, c_void_p,
@@ -273,10 +276,13 @@ def _host1d(x):
 
 class _PreFactor:
     """The tables behind get_propagation_pre_factor_function: pre (device, complex
-    (Nz, No) or (Nz, No, Nr)), renorm (device (Nz, No) or None)."""
+    (Nz, No) or (Nz, No, Nr)), renorm (device (Nz, No) or None).  A table error (a photon energy
+    outside the scattering-factor tables, or no f1 listed) raises here, or, with defer=True, in
+    check(): the tables are then only queued, so work queued after them does not wait for the
+    host."""
 
     def __init__(self, zgrid, rgrid, ogrid, preset_gas, pressure, absorption_tables, include_absorption,
-                 dispersion_tables, include_dispersion, n_ir):
+                 dispersion_tables, include_dispersion, n_ir, defer=False):
         if isinstance(pressure, dict) and 'zgrid' in pressure:
             # the tables are walked in one forward pass: ascending plane grid, strictly ascending
             # pressure grids (the reference's interp1d and cumulative_trapezoid assume the same)
@@ -333,11 +339,24 @@ class _PreFactor:
         shape = (nz, no, nr) if med.mode == 2 else (nz, no)
         self.pre = torch.empty(shape, dtype=torch.complex128, device="cuda")
         self.renorm = None if med.mode == 2 else torch.empty((nz, no), dtype=torch.float64, device="cuda")
-        _check(_L().hk_prefactor(byref(med), nz, _ptr(zg), no, _ptr(og), nr, _ptr(rg), _ptr(self.pre),
-                                 _ptr(self.renorm), _stream()))
+        self._flag = None
+        if defer:
+            self._flag = torch.empty(1, dtype=torch.int32, device="cuda")
+            self._work = torch.empty(no, dtype=torch.float64, device="cuda")
+            self._keep = keep + [zg, og, rg]  # what the queued kernels read
+            _check(_L().hk_prefactor_enqueue(byref(med), nz, _ptr(zg), no, _ptr(og), nr, _ptr(rg), _ptr(self.pre),
+                                             _ptr(self.renorm), _ptr(self._flag), _ptr(self._work), _stream()))
+        else:
+            _check(_L().hk_prefactor(byref(med), nz, _ptr(zg), no, _ptr(og), nr, _ptr(rg), _ptr(self.pre),
+                                     _ptr(self.renorm), _stream()))
         self.mode = med.mode
         self.nr = nr
         self._host = None
+
+    def check(self):
+        """Raises the deferred table error, if any (waits for the tables)."""
+        if self._flag is not None:
+            _check(_L().hk_prefactor_status(int(self._flag.item())))
 
     def host(self):
         if self._host is None:
@@ -516,7 +535,8 @@ class Hankel_long:
         if nz < 2:
             raise ValueError("Hankel_long needs at least two source planes")
         tab = _PreFactor(zgrid, target.rgrid, target.ogrid, preset_gas, pressure, absorption_tables,
-                         include_absorption, dispersion_tables, include_dispersion, effective_IR_refrective_index)
+                         include_absorption, dispersion_tables, include_dispersion, effective_IR_refrective_index,
+                         defer=True)
         src = target.device_planes()
         cumulative = store_cumulative_result or store_non_normalised_cumulative_result
         if store_cumulative_result and isinstance(pressure, dict) and 'rgrid' in pressure:
@@ -535,6 +555,7 @@ class Hankel_long:
             terms = 0.5 * torch.as_tensor(dz, device="cuda")[:, None, None] * (planes[:-1] + planes[1:])
             cum = torch.cumsum(terms, dim=0)
             ff, entry, exit_ = cum[-1], planes[0], planes[-1]
+            tab.check()  # before any result reaches the host
             if store_cumulative_result:
                 self.cumulative_field = (tab.renorm[:nz - 1][:, None, :] * cum).cpu().numpy()
             if store_non_normalised_cumulative_result:
@@ -546,6 +567,7 @@ class Hankel_long:
             else:
                 res = gridded(*args, W, near_field=near_field_factor, pre=tab.pre, kind=contraction)
             ff, entry, exit_ = res[0], res[1], res[2]
+            tab.check()  # before any result reaches the host
         self.FF_integrated = ff.cpu().numpy()
         if store_entry_and_exit_plane_transform:
             self.entry_plane_transform = entry.cpu().numpy()

@@ -72,6 +72,45 @@ def test_prefactor_tables(case):
         assert ren_f is None
 
 
+@pytest.mark.parametrize("case", ["scalar", "z", "zr"])
+def test_deferred_prefactor_equals_immediate(case):
+    """hk_prefactor_enqueue (tables queued, the table-error flag read later) gives the bits of
+    hk_prefactor; a table error surfaces in check(), and Hankel_long (which defers) raises it
+    before returning anything."""
+    from mma_amd import hankel as H
+    z = np.linspace(0.0, 5e-3, 17)
+    r = np.linspace(0.0, 1e-4, 9)
+    o = OMEGA0 * np.linspace(21, 61, 12)
+    if case == "scalar":
+        pressure = 0.6
+    elif case == "z":
+        zp = np.linspace(-1e-3, 6e-3, 57)
+        pressure = {"zgrid": zp, "value": 0.3 + 0.7 * np.exp(-((zp - 2e-3) / 1.5e-3) ** 2)}
+    else:
+        zp = np.linspace(0.0, 5e-3, 23)
+        rp = np.linspace(0.0, 1.2e-4, 7)
+        pressure = {"zgrid": zp, "rgrid": rp, "value": 0.5 + np.outer(np.sin(zp * 600), np.cos(rp * 1e4)) * 0.3}
+    args = (z, r, o, "Ar", pressure, "Henke", True, "NIST", True, 1.0002)
+    now = H._PreFactor(*args)
+    later = H._PreFactor(*args, defer=True)
+    later.check()
+    assert torch.equal(now.pre, later.pre)
+    assert (now.renorm is None and later.renorm is None) or torch.equal(now.renorm, later.renorm)
+    # 10.8 eV: Henke lists no f1 there
+    bad = (z, r, OMEGA0 * np.array([7.0, 9.0]), "Ar", pressure, "Henke", True, "Henke", True, 1.0)
+    with pytest.raises(ValueError):
+        H._PreFactor(*bad)
+    queued = H._PreFactor(*bad, defer=True)
+    with pytest.raises(ValueError, match="f1-scattering coefficient not listed"):
+        queued.check()
+    if case == "scalar":
+        S = np.ones((len(z), 2, len(r)), dtype=np.complex128)
+        tgt = H.FSources_provider(np.append(z, z[-1] + 1e-4), r, OMEGA0 * np.array([7.0, 9.0]), FSource=S)
+        with pytest.raises(ValueError, match="f1-scattering coefficient not listed"):
+            H.Hankel_long(tgt, 1.0, np.linspace(0, 1e-3, 5), "Ar", pressure, "Henke", True, "Henke", True, 1.0,
+                          contraction="fp32-mfma")
+
+
 def test_prefactor_z_modulated_closed_form():
     """The device z-modulated pre-factor on the piecewise-linear pressure table with planes
     before, inside (either side of the kink) and past the table, against its closed form (the
