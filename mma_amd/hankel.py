@@ -342,10 +342,10 @@ class _PreFactor:
         self._flag = None
         if defer:
             self._flag = torch.empty(1, dtype=torch.int32, device="cuda")
-            self._work = torch.empty(no, dtype=torch.float64, device="cuda")
+            self._exit_scratch = torch.empty(no, dtype=torch.float64, device="cuda")
             self._keep = keep + [zg, og, rg]  # what the queued kernels read
             _check(_L().hk_prefactor_enqueue(byref(med), nz, _ptr(zg), no, _ptr(og), nr, _ptr(rg), _ptr(self.pre),
-                                             _ptr(self.renorm), _ptr(self._flag), _ptr(self._work), _stream()))
+                                             _ptr(self.renorm), _ptr(self._flag), _ptr(self._exit_scratch), _stream()))
         else:
             _check(_L().hk_prefactor(byref(med), nz, _ptr(zg), no, _ptr(og), nr, _ptr(rg), _ptr(self.pre),
                                      _ptr(self.renorm), _stream()))
@@ -475,8 +475,8 @@ def gridded(src, zgrid, ogrid, rgrid, rho, distance, weights, near_field=True, p
         else:
             _check(L.hk_gemm(HK_GEMM_F32, p.nrho, Bm.shape[0], nq, _ptr(Am), nq, _ptr(Bm), nq, _ptr(out), Bm.shape[0],
                              _stream()))
-    out = out.view(p.nrho, n_out, p.no, 2).permute(1, 0, 2, 3).contiguous()
-    return torch.view_as_complex(out)
+    # a strided view (n_out, Nrho, No) of the GEMM's rows: no copy into output order
+    return torch.view_as_complex(out.view(p.nrho, n_out, p.no, 2).permute(1, 0, 2, 3))
 
 
 def HankelTransform(ogrid, rgrid, FField, distance, rgrid_FF, integrator=trapezoidal_integrator,
