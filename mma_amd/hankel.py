@@ -345,7 +345,8 @@ class _PreFactor:
             self._exit_scratch = torch.empty(no, dtype=torch.float64, device="cuda")
             self._keep = keep + [zg, og, rg]  # what the queued kernels read
             _check(_L().hk_prefactor_enqueue(byref(med), nz, _ptr(zg), no, _ptr(og), nr, _ptr(rg), _ptr(self.pre),
-                                             _ptr(self.renorm), _ptr(self._flag), _ptr(self._exit_scratch), _stream()))
+                                             _ptr(self.renorm), _ptr(self._flag), _ptr(self._exit_scratch),
+                                             _stream()))
         else:
             _check(_L().hk_prefactor(byref(med), nz, _ptr(zg), no, _ptr(og), nr, _ptr(rg), _ptr(self.pre),
                                      _ptr(self.renorm), _stream()))
