@@ -207,6 +207,9 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
     const long long base = tdse_chunk_base(cl);
     const int par = seg & 1;
 
+    // raised priority while this wave issues the segment's loads, so that waves with loads to
+    // issue go ahead of waves that only compute
+    __builtin_amdgcn_s_setprio(2);
     SegTables T;
     if constexpr (STASH && MODE == kFromRegs) {
         load_tables<true>(G, cs, wave, lane, nw, T, stash);
@@ -242,6 +245,7 @@ __device__ void solve_segment(const TdseGeom &G, int seg, int own_chunks, int wa
             nb = q < own_chunks ? pl->bnd[par][wave + 1][0] : pl->halo[par ^ 1][q - own_chunks][0];
         }
     }
+    __builtin_amdgcn_s_setprio(0);
     const bool own = MODE == kParallel
                          ? gl >= G.halo_lanes && gl < G.halo_lanes + own_chunks && c >= 0 && c < G.nchunks
                          : gl < own_chunks && c < G.nchunks;
