@@ -36,10 +36,11 @@ sys.path.insert(0, ROOT)
 METRIC = "TDSE grid-pt-steps/s batched vs per-instance @≤1e-8 err; adjoint/fwd time ratio"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PT_STEP = 32.0  # psi read + write, complex128
-# HBM bytes per launch of tdse_propagate_kernel on the default 64 x 64 workload, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE counters scaled by their readings for a plain 1 GiB device copy
-# (x2.0 and x1.0); commands and readings in profiles/tdse_propagate_gaussian_beam_4096.md
-MEASURED_TRAFFIC_BYTES = 5.105e12 + 4.685e12
+# HBM bytes per launch on the default 64 x 64 workload, by steps per pass of the launch shape
+# (1: tdse_propagate_kernel at 4 x 3, 2: tdse_propagate_pair_stash_kernel at 4 x 2), from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by their readings for a plain 1 GiB device
+# copy (x2.0 and x1.0); commands and readings in profiles/tdse_propagate_gaussian_beam_4096.md
+MEASURED_TRAFFIC_BYTES = {1: 5.105e12 + 4.685e12, 2: 2.636e12 + 2.394e12}
 
 
 def log(*a):
@@ -701,7 +702,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "copy_ceiling": copy_gbs, "frac_of_copy_ceiling": achieved / copy_gbs if copy_gbs else None,
                      "read_only_ceiling": read_gbs, "write_only_ceiling": write_gbs,
-                     "traffic": MEASURED_TRAFFIC_BYTES if (args.n_r, args.n_z) == (64, 64) else None},
+                     "traffic": MEASURED_TRAFFIC_BYTES.get(tuned.config.as_tuple()[2])
+                     if (args.n_r, args.n_z) == (64, 64) else None,
+                     "kernel": "tdse_propagate_pair_stash_kernel" if tuned.config.as_tuple()[2] == 2
+                     else "tdse_propagate_kernel"},
         "cpu_baseline": cpu,
         "gpu_over_cpu_baseline": (value / cpu["value"]) if cpu else None,
         "parity_vs_reference": parity,
