@@ -66,6 +66,33 @@ def test_shared_lattice_tables_match_own_tables():
             rel_err(a[i, 0, :steps[i]], b[i, 0, :steps[i]]) for i in range(len(sizes))), flush=True)
 
 
+def test_instance_queues_give_every_instance_its_own_bits():
+    """The persistent launch takes instances from one queue per XCD: members of a shared lattice
+    by their alignment (here all in one alignment, so the other queues' workgroups take them
+    from that queue), the rest dealt round-robin.  Every instance of a batch mixing both gets
+    the bits it gets alone (same dx and dt, so the same halo and launch shape)."""
+    import os
+    lattice = tdse.Geometry.create_many([3000, 2984, 2968, 2952], 0.1, 0.25, W.GRID["a"], W.GRID["e_guess"],
+                                        1e-25, 2.0)
+    own = [_grid_geometry(n) for n in (2001, 1600)]
+    geoms = [lattice[i % 4] for i in range(12)] + [own[i % 2] for i in range(6)]
+    assert len({g.halo_points for g in geoms}) == 1
+    rng = np.random.default_rng(33)
+    t = np.arange(300) * 0.25
+    f = np.stack([W.sin2_pulse(t, rng.uniform(0.04, 0.09), W.omega_au(800e-9), 3, phase=rng.uniform(0, 3))
+                  for _ in geoms])
+    dev = torch.tensor(tdse.pad_fields(f), device="cuda")
+    for spp in ("1", "2"):
+        os.environ["MMA_TDSE_STEPS_PER_PASS"] = spp
+        try:
+            batch = tdse.propagate_batch(geoms, dev).cpu().numpy()
+            for i, g in enumerate(geoms):
+                alone = tdse.propagate_batch([g], dev[i:i + 1]).cpu().numpy()
+                np.testing.assert_array_equal(batch[i], alone[0], err_msg="instance %d, steps/pass %s" % (i, spp))
+        finally:
+            del os.environ["MMA_TDSE_STEPS_PER_PASS"]
+
+
 def test_create_many_reports_bad_members():
     with pytest.raises(tdse.CTDSEError):
         tdse.Geometry.create_many([1000, 1], 0.1, 0.25, W.GRID["a"])
