@@ -231,6 +231,31 @@ def test_hankel_long_gridded(contraction, tol):
         assert e < tol, name
 
 
+def test_float_spreads_at_a_large_near_field_phase():
+    """The two operands whose spread runs in float (two bf16 parts, two scaled fp16 parts) with
+    the screen at 5 cm, where the near-field phase c r^2 reaches ~190 rad: the phases anchor on
+    an FP64 sincos (a float argument would be off by ulp(190) = 1.5e-5 rad up to the next
+    anchor), so the errors against the FP64 direct contraction stay those of the operands:
+    1e-6 for the fp16 parts (22 bits; measured 1.3e-7), and for the bf16 parts the 2^-17
+    relative rounding of an operand element, 7.6e-6 (measured up to 3.6e-6 on one plane)."""
+    from mma_amd import hankel as H
+    zall, r, o, S, pressure, rho = _long_case(nz=24, no=16, nr=200)
+    distance = 0.05
+    rho = rho * distance
+    ref = None
+    for contraction, tol in (("fp64", None), ("fp16x3-mfma", 1e-6), ("bf16x3-mfma", 7.6e-6)):
+        tgt = H.FSources_provider(zall, r, o, FSource=S)
+        res = H.Hankel_long(tgt, distance, rho, "Ar", pressure, "Henke", True, "NIST", True, 1.0001,
+                            contraction=contraction)
+        if ref is None:
+            ref = res
+            continue
+        for name in ("FF_integrated", "entry_plane_transform", "exit_plane_transform"):
+            e = rel_err(getattr(res, name), getattr(ref, name))
+            print("5 cm", contraction, name, "max %.2e" % e, flush=True)
+            assert e < tol, (contraction, name, e)
+
+
 def test_hankel_long_gridded_nonuniform_r_and_scalar_pressure():
     """The gridded contraction on a non-uniform radial grid (trapezoid weights of the given
     r, bins of q = K r / D at arbitrary r) with the scalar-pressure pre-factor, and with an odd
