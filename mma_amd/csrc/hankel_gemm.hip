@@ -252,11 +252,10 @@ __device__ inline void split_gemm(int M, int N, int K, const unsigned short *A, 
         }
 
     const int stages = K / S::SBK;
-    issue(0, 0);
-    for (int s = 0; s < stages; s++) {
-        const int buf = s & 1;
+    // stage s's copies are in LDS for every wave once this returns
+    auto stage_ready = [&](int s) {
         if (s + 1 < stages) {
-            issue((s + 1) * S::SBK, buf ^ 1);
+            issue((s + 1) * S::SBK, (s + 1) & 1);
             // this wave's copies of stage s are done when only stage s + 1's remain
             if constexpr (S::PER_WAVE == 8)
                 asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
@@ -268,17 +267,84 @@ __device__ inline void split_gemm(int M, int N, int K, const unsigned short *A, 
         // every wave's copies of stage s are in LDS (a barrier with a memory clobber: no LDS
         // access moves across it)
         asm volatile("s_barrier" ::: "memory");
+    };
+    issue(0, 0);
+    if constexpr (!F16) {
+        // Two sets of FP32 sums: a fold group (S::FOLD stages, 64 K terms) accumulates in one
+        // while the previous group's sums, in the other, are added to the FP64 sums between its
+        // MFMAs instead of after them.  The FP64 sums get the same group sums in the same order
+        // as when each group is folded at its end.
+        f32x16 other[2];
 #pragma unroll
-        for (int kk = 0; kk < S::SBK; kk += 16) {
-            const int c = kk / 8 + kh;
-            op8 ap[NP], bp[NP][2];
+        for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-            for (int q = 0; q < NP; q++) {
-                ap[q] = operand(buf, q, wm * 32 + r32, c);
+            for (int v = 0; v < 16; v++) other[tj][v] = 0.0f;
+        constexpr int KK = S::SBK / 16;                   // MFMA K steps per stage
+        constexpr int PER_STEP = 32 / KK;                 // sums folded per K step (first stage)
+        auto group = [&](int g, f32x16 (&use)[2], f32x16 (&old)[2]) {
 #pragma unroll
-                for (int t = 0; t < 2; t++) bp[q][t] = operand(buf, q, BM + wn * 64 + t * 32 + r32, c);
+            for (int f = 0; f < S::FOLD; f++) {
+                const int s = g * S::FOLD + f, buf = s & 1;
+                stage_ready(s);
+#pragma unroll
+                for (int kk = 0; kk < KK; kk++) {
+                    const int c = 2 * kk + kh;
+                    op8 ap[NP], bp[NP][2];
+#pragma unroll
+                    for (int q = 0; q < NP; q++) {
+                        ap[q] = operand(buf, q, wm * 32 + r32, c);
+#pragma unroll
+                        for (int t = 0; t < 2; t++) bp[q][t] = operand(buf, q, BM + wn * 64 + t * 32 + r32, c);
+                    }
+#pragma unroll
+                    for (int tj = 0; tj < 2; tj++) {
+                        f32x16 v = use[tj];
+                        if constexpr (NP == 3) {  // smallest terms first
+                            v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[2], bp[0][tj], v, 0, 0, 0);
+                            v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1], bp[1][tj], v, 0, 0, 0);
+                            v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[2][tj], v, 0, 0, 0);
+                        }
+                        v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1], bp[0][tj], v, 0, 0, 0);
+                        v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[1][tj], v, 0, 0, 0);
+                        v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[0][tj], v, 0, 0, 0);
+                        use[tj] = v;
+                    }
+                    if (f == 0) {
+#pragma unroll
+                        for (int e = kk * PER_STEP; e < (kk + 1) * PER_STEP; e++) {
+                            accd[e / 16][e % 16] += (double)old[e / 16][e % 16];
+                            old[e / 16][e % 16] = 0.0f;
+                        }
+                    }
+                }
+                asm volatile("s_barrier" ::: "memory");  // buffer `buf` is free for stage s + 2
             }
-            if constexpr (F16) {
+        };
+        const int groups = stages / S::FOLD;  // K is a multiple of 64
+        for (int g = 0; g < groups; g += 2) {
+            group(g, acc, other);
+            if (g + 1 < groups) group(g + 1, other, acc);
+        }
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) accd[tj][v] += (double)acc[tj][v] + (double)other[tj][v];
+    } else {
+        // fp16 parts: the cross terms on a second set of sums; with no registers left for two
+        // more sets, each group is folded at its end
+        for (int s = 0; s < stages; s++) {
+            const int buf = s & 1;
+            stage_ready(s);
+#pragma unroll
+            for (int kk = 0; kk < S::SBK; kk += 16) {
+                const int c = kk / 8 + kh;
+                op8 ap[NP], bp[NP][2];
+#pragma unroll
+                for (int q = 0; q < NP; q++) {
+                    ap[q] = operand(buf, q, wm * 32 + r32, c);
+#pragma unroll
+                    for (int t = 0; t < 2; t++) bp[q][t] = operand(buf, q, BM + wn * 64 + t * 32 + r32, c);
+                }
 #pragma unroll
                 for (int tj = 0; tj < 2; tj++) {
                     f32x16 x = acc1[tj];
@@ -287,36 +353,20 @@ __device__ inline void split_gemm(int M, int N, int K, const unsigned short *A, 
                     acc1[tj] = x;
                     acc[tj] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ap[0], bp[0][tj], acc[tj], 0, 0, 0);
                 }
-                continue;
             }
+            if ((s + 1) % S::FOLD == 0 || s + 1 == stages) {
 #pragma unroll
-            for (int tj = 0; tj < 2; tj++) {
-                f32x16 v = acc[tj];
-                if constexpr (NP == 3) {  // smallest terms first
-                    v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[2], bp[0][tj], v, 0, 0, 0);
-                    v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1], bp[1][tj], v, 0, 0, 0);
-                    v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[2][tj], v, 0, 0, 0);
-                }
-                v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1], bp[0][tj], v, 0, 0, 0);
-                v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[1][tj], v, 0, 0, 0);
-                v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0], bp[0][tj], v, 0, 0, 0);
-                acc[tj] = v;
-            }
-        }
-        if ((s + 1) % S::FOLD == 0 || s + 1 == stages) {
+                for (int tj = 0; tj < 2; tj++)
 #pragma unroll
-            for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    accd[tj][v] += (double)acc[tj][v];
-                    acc[tj][v] = 0.0f;
-                    if constexpr (F16) {
+                    for (int v = 0; v < 16; v++) {
+                        accd[tj][v] += (double)acc[tj][v];
+                        acc[tj][v] = 0.0f;
                         accd[tj][v] += 0x1p-11 * (double)acc1[tj][v];
                         acc1[tj][v] = 0.0f;
                     }
-                }
+            }
+            asm volatile("s_barrier" ::: "memory");  // buffer `buf` is free for stage s + 2
         }
-        asm volatile("s_barrier" ::: "memory");  // buffer `buf` is free for stage s + 2
     }
 #pragma unroll
     for (int tj = 0; tj < 2; tj++) {
