@@ -180,6 +180,9 @@ template __global__ void hk_gemm_kernel<HK_GEMM_BF16X6>(int, int, int, const flo
 // latency hidden behind the previous stage's MFMAs.  Tile rows (A rows, then B rows) are
 // SBK bf16 long; the 16-byte chunk c of row r sits at slot c ^ ((r / RPB) & (CH - 1)), RPB the
 // rows in 256 bytes, so the 32 rows an MFMA operand read touches spread over all banks.
+// The barrier that ends a stage first waits for the wave's own LDS reads (lgkmcnt): the
+// compiler may leave an operand read in flight across an asm barrier, and the copies issued
+// after it overwrite that buffer.
 // one 64-lane copy of 16 bytes per lane from global memory to LDS (lane i writes lds + 16 i);
 // a device function of its own: the builtin exists only in the device compilation
 __device__ inline void copy_to_lds(const unsigned short *g, unsigned short *lds)
@@ -317,7 +320,7 @@ __device__ inline void split_gemm(int M, int N, int K, const unsigned short *A, 
                         }
                     }
                 }
-                asm volatile("s_barrier" ::: "memory");  // buffer `buf` is free for stage s + 2
+                asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");  // `buf` free for stage s + 2
             }
         };
         const int groups = stages / S::FOLD;  // K is a multiple of 64
@@ -365,7 +368,7 @@ __device__ inline void split_gemm(int M, int N, int K, const unsigned short *A, 
                         acc1[tj][v] = 0.0f;
                     }
             }
-            asm volatile("s_barrier" ::: "memory");  // buffer `buf` is free for stage s + 2
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");  // `buf` free for stage s + 2
         }
     }
 #pragma unroll
