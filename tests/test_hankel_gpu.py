@@ -337,11 +337,13 @@ def _bf16_parts(x, n):
 def test_split_gemm_matches_fp64():
     """hk_gemm_split on operands split on the host: the products of the kept part pairs (two
     parts: p0 q0 + p0 q1 + p1 q0; three: the six of order <= 2^-16) against their FP64 sum,
-    with ragged m and n (tile edges) and the far-field workload's K = 5120."""
+    with ragged m and n (tile edges), the far-field workload's K = 5120, and K = 64 and 192: one
+    and three groups of 64 K terms, whose FP32 sums alternate between two register sets (an odd
+    count ends on the first set)."""
     from mma_amd import hankel as H
     rng = np.random.default_rng(11)
     pairs = {2: [(0, 0), (0, 1), (1, 0)], 3: [(0, 0), (0, 1), (1, 0), (0, 2), (1, 1), (2, 0)]}
-    for m, n, k in ((200, 330, 256), (1024, 600, 5120)):
+    for m, n, k in ((200, 330, 256), (130, 140, 64), (129, 257, 192), (1024, 600, 5120)):
         A, B = rng.standard_normal((m, k)), rng.standard_normal((n, k))
         for parts in (2, 3):
             ap, bp = _bf16_parts(A, parts), _bf16_parts(B, parts)
