@@ -36,6 +36,18 @@ __device__ inline void st2(double2 *p, long long i, double2 v)
 }
 __device__ inline double ld1(const double *p, long long i) { return ((const TDSE_GLOBAL double *)p)[i]; }
 
+// p + i kept opaque to the compiler: the loads at constant distances from it (the C rows of a
+// chunk, 64 elements apart) then take the distance as the instruction's immediate offset.
+// Written as base + 64 k, whose low bits the compiler knows to be clear, the sum becomes an OR
+// and every load gets its own 64-bit address (four VALU instructions each).
+template <typename T>
+__device__ inline const T *lane_ptr(const T *p, long long i)
+{
+    unsigned long long a = reinterpret_cast<unsigned long long>(p + i);
+    asm("" : "+v"(a));
+    return reinterpret_cast<const T *>(a);
+}
+
 // Workgroup barrier for exchanges through LDS only: waits for this wave's LDS accesses but
 // not for its global loads and stores (a segment's psi stores stay in flight across the next
 // segment's barriers).  Global memory written by one wave and read by another needs
