@@ -82,8 +82,8 @@ struct SegTables {
 
 // IB_LDS: 1/beta from `stash` (point i of lane gl at stash[i * 64 nw + gl]) instead of HBM.
 // ROW_PTRS: address the C rows of a table from one opaque lane pointer (lane_ptr), each row an
-// immediate offset: 1-1.6 % faster in the one-step kernel, 0.5 % slower in the two-step ones,
-// which keep the plain indexing.
+// immediate offset: 1-1.6 % faster in the one-step kernel, 0.5 % slower in the two-step ones
+// (3.6 % in the adjoint sweep), which keep the plain indexing.
 template <bool IB_LDS = false, bool ROW_PTRS = false>
 __device__ inline void load_tables(const TdseGeom &G, int cs, int wave, int lane, int nw, SegTables &T,
                                    const double2 *stash = nullptr)
